@@ -1,0 +1,176 @@
+/*
+ * libopenblas_amd -- MI355X (gfx950) implementation of OpenBLAS's level-3
+ * hot path behind OpenBLAS's own C ABI.
+ *
+ * Every function below except the `openblas_amd_*` extensions has exactly the
+ * name, argument list, argument meaning, info numbering and error behaviour
+ * of the OpenBLAS entry point it replaces; the comment above each names that
+ * reference interface (paths relative to the OpenBLAS source tree).
+ *
+ * Matrices are column-major (unless a CBLAS/LAPACKE layout argument says
+ * otherwise) and may live in host memory or in device memory allocated with
+ * hipMalloc/openblas_amd_malloc: host arguments are staged to the GPU and
+ * back, device arguments are used in place.  Every call is synchronous.
+ * blasint is a 32-bit int (LP64).  There is no CPU fallback: without a
+ * gfx950 device the first call aborts with a message.
+ */
+#ifndef OPENBLAS_AMD_H
+#define OPENBLAS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int blasint;
+
+/* enum values are ABI facts of cblas.h:62-66 */
+typedef enum CBLAS_ORDER     {CblasRowMajor = 101, CblasColMajor = 102} CBLAS_ORDER;
+typedef enum CBLAS_TRANSPOSE {CblasNoTrans = 111, CblasTrans = 112, CblasConjTrans = 113,
+                              CblasConjNoTrans = 114} CBLAS_TRANSPOSE;
+typedef enum CBLAS_UPLO      {CblasUpper = 121, CblasLower = 122} CBLAS_UPLO;
+typedef enum CBLAS_DIAG      {CblasNonUnit = 131, CblasUnit = 132} CBLAS_DIAG;
+typedef enum CBLAS_SIDE      {CblasLeft = 141, CblasRight = 142} CBLAS_SIDE;
+
+#define LAPACK_ROW_MAJOR 101
+#define LAPACK_COL_MAJOR 102
+
+/* ---- CBLAS level 3 -------------------------------------------------------- */
+/* cblas.h:298-300, interface/gemm.c:294 (CNAME built with -DCBLAS) */
+void cblas_dgemm(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                 blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
+                 const double *B, blasint ldb, double beta, double *C, blasint ldc);
+void cblas_sgemm(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                 blasint M, blasint N, blasint K, float alpha, const float *A, blasint lda,
+                 const float *B, blasint ldb, float beta, float *C, blasint ldc);
+/* cblas.h:356-358, interface/trsm.c:213 */
+void cblas_dtrsm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo,
+                 enum CBLAS_TRANSPOSE TransA, enum CBLAS_DIAG Diag, blasint M, blasint N,
+                 double alpha, const double *A, blasint lda, double *B, blasint ldb);
+void cblas_strsm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo,
+                 enum CBLAS_TRANSPOSE TransA, enum CBLAS_DIAG Diag, blasint M, blasint N,
+                 float alpha, const float *A, blasint lda, float *B, blasint ldb);
+/* cblas.h:419-422, interface/gemm_batch.c:117 (group API) */
+void cblas_dgemm_batch(enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE *TransA_array,
+                       const enum CBLAS_TRANSPOSE *TransB_array, const blasint *M_array,
+                       const blasint *N_array, const blasint *K_array, const double *alpha_array,
+                       const double **A_array, const blasint *lda_array, const double **B_array,
+                       const blasint *ldb_array, const double *beta_array, double **C_array,
+                       const blasint *ldc_array, blasint group_count, const blasint *group_size);
+void cblas_sgemm_batch(enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE *TransA_array,
+                       const enum CBLAS_TRANSPOSE *TransB_array, const blasint *M_array,
+                       const blasint *N_array, const blasint *K_array, const float *alpha_array,
+                       const float **A_array, const blasint *lda_array, const float **B_array,
+                       const blasint *ldb_array, const float *beta_array, float **C_array,
+                       const blasint *ldc_array, blasint group_count, const blasint *group_size);
+
+/* ---- Fortran BLAS level 3 (BLASFUNC(x) = x##_, common.h:194) -------------- */
+/* interface/gemm.c:181 */
+void dgemm_(const char *transa, const char *transb, const blasint *m, const blasint *n, const blasint *k,
+            const double *alpha, const double *a, const blasint *lda, const double *b, const blasint *ldb,
+            const double *beta, double *c, const blasint *ldc);
+void sgemm_(const char *transa, const char *transb, const blasint *m, const blasint *n, const blasint *k,
+            const float *alpha, const float *a, const blasint *lda, const float *b, const blasint *ldb,
+            const float *beta, float *c, const blasint *ldc);
+/* interface/trsm.c:114 */
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const blasint *m,
+            const blasint *n, const double *alpha, const double *a, const blasint *lda, double *b,
+            const blasint *ldb);
+void strsm_(const char *side, const char *uplo, const char *transa, const char *diag, const blasint *m,
+            const blasint *n, const float *alpha, const float *a, const blasint *lda, float *b,
+            const blasint *ldb);
+
+/* ---- LAPACK (Fortran ABI) --------------------------------------------------- */
+/* interface/lapack/getrf.c:53 */
+int dgetrf_(const blasint *m, const blasint *n, double *a, const blasint *lda, blasint *ipiv, blasint *info);
+int sgetrf_(const blasint *m, const blasint *n, float *a, const blasint *lda, blasint *ipiv, blasint *info);
+/* interface/lapack/getf2.c (same contract as getrf, unblocked in the reference) */
+int dgetf2_(const blasint *m, const blasint *n, double *a, const blasint *lda, blasint *ipiv, blasint *info);
+int sgetf2_(const blasint *m, const blasint *n, float *a, const blasint *lda, blasint *ipiv, blasint *info);
+/* interface/lapack/getrs.c:63 */
+int dgetrs_(const char *trans, const blasint *n, const blasint *nrhs, const double *a, const blasint *lda,
+            const blasint *ipiv, double *b, const blasint *ldb, blasint *info);
+int sgetrs_(const char *trans, const blasint *n, const blasint *nrhs, const float *a, const blasint *lda,
+            const blasint *ipiv, float *b, const blasint *ldb, blasint *info);
+/* interface/lapack/gesv.c:63 */
+int dgesv_(const blasint *n, const blasint *nrhs, double *a, const blasint *lda, blasint *ipiv, double *b,
+           const blasint *ldb, blasint *info);
+int sgesv_(const blasint *n, const blasint *nrhs, float *a, const blasint *lda, blasint *ipiv, float *b,
+           const blasint *ldb, blasint *info);
+/* interface/lapack/potrf.c:63 */
+int dpotrf_(const char *uplo, const blasint *n, double *a, const blasint *lda, blasint *info);
+int spotrf_(const char *uplo, const blasint *n, float *a, const blasint *lda, blasint *info);
+/* interface/lapack/potf2.c */
+int dpotf2_(const char *uplo, const blasint *n, double *a, const blasint *lda, blasint *info);
+int spotf2_(const char *uplo, const blasint *n, float *a, const blasint *lda, blasint *info);
+/* interface/lapack/laswp.c:55 */
+int dlaswp_(const blasint *n, double *a, const blasint *lda, const blasint *k1, const blasint *k2,
+            const blasint *ipiv, const blasint *incx);
+int slaswp_(const blasint *n, float *a, const blasint *lda, const blasint *k1, const blasint *k2,
+            const blasint *ipiv, const blasint *incx);
+/* lapack-netlib/SRC/dsgesv.f:193 (WORK and SWORK are accepted and not used) */
+int dsgesv_(const blasint *n, const blasint *nrhs, double *a, const blasint *lda, blasint *ipiv,
+            const double *b, const blasint *ldb, double *x, const blasint *ldx, double *work,
+            float *swork, blasint *iter, blasint *info);
+
+/* ---- LAPACKE (lapack-netlib/LAPACKE/include/lapacke.h) ---------------------- */
+/* lapacke.h:872 and lapacke_dgesv_work.c:35 */
+int LAPACKE_dgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb);
+int LAPACKE_dgesv_work(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb);
+int LAPACKE_sgesv(int matrix_layout, int n, int nrhs, float *a, int lda, int *ipiv, float *b, int ldb);
+int LAPACKE_sgesv_work(int matrix_layout, int n, int nrhs, float *a, int lda, int *ipiv, float *b, int ldb);
+/* lapacke.h:883 */
+int LAPACKE_dsgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb,
+                   double *x, int ldx, int *iter);
+/* lapacke.h:1062-1064 */
+int LAPACKE_dgetrf(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
+int LAPACKE_dgetrf_work(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
+int LAPACKE_sgetrf(int matrix_layout, int m, int n, float *a, int lda, int *ipiv);
+int LAPACKE_sgetrf_work(int matrix_layout, int m, int n, float *a, int lda, int *ipiv);
+/* lapacke.h:1098 */
+int LAPACKE_dgetrs(int matrix_layout, char trans, int n, int nrhs, const double *a, int lda, const int *ipiv,
+                   double *b, int ldb);
+int LAPACKE_dgetrs_work(int matrix_layout, char trans, int n, int nrhs, const double *a, int lda,
+                        const int *ipiv, double *b, int ldb);
+int LAPACKE_sgetrs(int matrix_layout, char trans, int n, int nrhs, const float *a, int lda, const int *ipiv,
+                   float *b, int ldb);
+int LAPACKE_sgetrs_work(int matrix_layout, char trans, int n, int nrhs, const float *a, int lda,
+                        const int *ipiv, float *b, int ldb);
+/* lapacke.h:3055-3057 */
+int LAPACKE_dpotrf(int matrix_layout, char uplo, int n, double *a, int lda);
+int LAPACKE_dpotrf_work(int matrix_layout, char uplo, int n, double *a, int lda);
+int LAPACKE_spotrf(int matrix_layout, char uplo, int n, float *a, int lda);
+int LAPACKE_spotrf_work(int matrix_layout, char uplo, int n, float *a, int lda);
+
+/* ---- runtime / informational (cblas.h:13-27, driver/others/xerbla.c:56) ----- */
+int xerbla_(char *name, blasint *info, blasint len);   /* weak: applications may override */
+char *openblas_get_config(void);
+char *openblas_get_corename(void);
+void openblas_set_num_threads(int num_threads);
+int openblas_get_num_threads(void);
+int openblas_get_num_procs(void);
+
+/* ---- extensions: no counterpart in the reference ---------------------------- */
+/* device memory and synchronisation for callers that keep matrices in HBM */
+void *openblas_amd_malloc(size_t bytes);
+void openblas_amd_free(void *p);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int openblas_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int openblas_amd_memset(void *dst, int value, size_t bytes);
+int openblas_amd_synchronize(void);
+void openblas_amd_set_sync(int on);
+int openblas_amd_device_count(void);
+int openblas_amd_device_info(char *arch, int len, int *num_cu, int *clock_khz);
+/* HIP events on the library's stream: brackets exactly the library's kernels */
+void openblas_amd_timer_start(void);
+float openblas_amd_timer_stop(void);
+/* last argument error reported through xerbla on this thread (0 if none);
+   copies the routine name; clears the record */
+int openblas_amd_last_xerbla(char *name, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,286 @@
+// Level-3 GEMM for gfx950: C <- alpha*op(A)*op(B) + beta*C, column-major,
+// FP64 on v_mfma_f64_16x16x4f64 and FP32 on v_mfma_f32_16x16x4f32.
+//
+// Reference semantics followed (not its blocking): the β pre-pass and the
+// k==0 / α==0 early exit of driver/level3/level3.c:229-264, β==0 writing
+// exact zeros (kernel/generic/gemm_beta.c:52), all four NN/NT/TN/TT forms of
+// interface/gemm.c:87-111.
+//
+// Design (MI355X, not the GotoBLAS 5-loop): one workgroup of 4 wavefronts
+// owns a BM x BN tile of C held in MFMA accumulators; K advances in BK slabs
+// that are loaded from HBM with 16-byte vector loads into registers one slab
+// ahead and staged through a double-buffered k-major LDS tile (padded so the
+// fragment reads are bank-conflict free).  The MFMA is issued with op(B) as
+// its first operand so that the 16 consecutive lanes of an output row map to
+// 16 consecutive rows of column-major C (128-byte coalesced stores).  Tiles
+// are mapped XCD-aware: the 8 XCDs each get a contiguous, grouped block of C
+// tiles so that the panels of A and B they share stay in their own L2.
+#include "kernels.h"
+
+namespace oba {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MF;
+template <> struct MF<double> {
+  typedef d4 acc;
+  __device__ static inline acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // accumulator element r of lane l holds output row 4r + l/16 (measured
+  // lane layout of the f64 16x16x4 instruction), column l%16.
+  __device__ static inline int orow(int l, int r) { return 4 * r + (l >> 4); }
+};
+template <> struct MF<float> {
+  typedef f4 acc;
+  __device__ static inline acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int orow(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T> struct Vec16;
+template <> struct Vec16<double> { typedef double2 type; static constexpr int n = 2; };
+template <> struct Vec16<float> { typedef float4 type; static constexpr int n = 4; };
+
+// Loads one BK-deep slab of an operand tile (TILE along m or n) into
+// registers.  KC: the operand is contiguous along k in memory (Trans A /
+// NoTrans B).  VEC: 16-byte vector loads are legal (aligned base and ld).
+template <typename T, int TILE, int BK, bool KC, bool VEC>
+struct TileLoader {
+  static constexpr int V = VEC ? Vec16<T>::n : 1;
+  static constexpr int NV = TILE * BK / V / 256;  // vectors per thread
+  static_assert(NV * V * 256 == TILE * BK, "tile must divide over 256 threads");
+  T r[NV * V];
+
+  __device__ inline void load(const T* __restrict__ g, long ld, long mn0, long k0, long MN, long K) {
+    const int tid = threadIdx.x;
+    const bool full = (mn0 + TILE <= MN) && (k0 + BK <= K);
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      const int v = tid + i * 256;
+      long mn, k;
+      if (!KC) { k = v / (TILE / V); mn = (v % (TILE / V)) * V; }
+      else { mn = v / (BK / V); k = (v % (BK / V)) * V; }
+      mn += mn0; k += k0;
+      if (full) {
+        if constexpr (V > 1) {
+          const T* p = KC ? g + k + mn * ld : g + mn + k * ld;
+          typedef typename Vec16<T>::type vt;
+          vt x = *(const vt*)p;
+          const T* xs = (const T*)&x;
+#pragma unroll
+          for (int e = 0; e < V; e++) r[i * V + e] = xs[e];
+        } else {
+          r[i] = KC ? g[k + mn * ld] : g[mn + k * ld];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; e++) {
+          long mm = KC ? mn : mn + e, kk = KC ? k + e : k;
+          r[i * V + e] = (mm < MN && kk < K) ? g[KC ? kk + mm * ld : mm + kk * ld] : T(0);
+        }
+      }
+    }
+  }
+  // writes the slab into a k-major LDS tile s[BK][TILE + PAD]
+  template <int LDT>
+  __device__ inline void store(T* s) const {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      const int v = tid + i * 256;
+      if (!KC) {
+        int k = v / (TILE / V), mn = (v % (TILE / V)) * V;
+        if constexpr (V > 1) {
+          typedef typename Vec16<T>::type vt;
+          vt x;
+          T* xs = (T*)&x;
+#pragma unroll
+          for (int e = 0; e < V; e++) xs[e] = r[i * V + e];
+          *(vt*)(s + k * LDT + mn) = x;
+        } else {
+          s[k * LDT + mn] = r[i];
+        }
+      } else {
+        int mn = v / (BK / V), k = (v % (BK / V)) * V;
+#pragma unroll
+        for (int e = 0; e < V; e++) s[(k + e) * LDT + mn] = r[i * V + e];
+      }
+    }
+  }
+};
+
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T alpha,
+                                                      const T* __restrict__ A, long lda,
+                                                      const T* __restrict__ B, long ldb, T beta,
+                                                      T* __restrict__ C, long ldc, int tiles_m,
+                                                      int tiles_n, int tri, const int* abort_flag) {
+  constexpr int PAD = 64 / sizeof(T) * (sizeof(T) == 8 ? 2 : 1);  // 16 doubles / 16 floats
+  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  constexpr int WTM = BM / 2, WTN = BN / 2;  // 2x2 waves
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  typedef MF<T> mf;
+  typedef typename mf::acc acc_t;
+
+  __shared__ T smem[2 * BK * (LDA_S + LDB_S)];
+  // buffer b of the A slab at smem + b*BK*LDA_S, of the B slab after both A's
+#define OBA_AS(b) (smem + (b) * BK * LDA_S)
+#define OBA_BS(b) (smem + 2 * BK * LDA_S + (b) * BK * LDB_S)
+
+  // XCD-aware tile order: workgroup b runs on XCD b%8; give each XCD a
+  // contiguous range of the grouped tile order.
+  const int T_all = tiles_m * tiles_n;
+  int lin = blockIdx.x;
+  if ((T_all & 7) == 0) lin = (blockIdx.x & 7) * (T_all >> 3) + (blockIdx.x >> 3);
+  constexpr int GM = 8;
+  const int group = lin / (GM * tiles_n);
+  const int first_m = group * GM;
+  const int gsize = min(tiles_m - first_m, GM);
+  const int in_group = lin - group * GM * tiles_n;
+  const int tm = first_m + in_group % gsize;
+  const int tn = in_group / gsize;
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  // tri: 1 = update only C(i,j) with i >= j (lower), 2 = only i <= j (upper)
+  if (tri == 1 && n0 > m0 + BM - 1) return;
+  if (tri == 2 && m0 > n0 + BN - 1) return;
+  if (abort_flag && *abort_flag) return;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WTM, wn0 = (wave >> 1) * WTN;
+
+  TileLoader<T, BM, BK, TA, VEC> la;
+  TileLoader<T, BN, BK, !TB, VEC> lb;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; i++)
+#pragma unroll
+    for (int j = 0; j < FN; j++) acc[i][j] = acc_t{};
+
+  const int nk = (int)((K + BK - 1) / BK);
+  la.load(A, lda, m0, 0, M, K);
+  lb.load(B, ldb, n0, 0, N, K);
+  la.template store<LDA_S>(OBA_AS(0));
+  lb.template store<LDB_S>(OBA_BS(0));
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      la.load(A, lda, m0, (long)(kt + 1) * BK, M, K);
+      lb.load(B, ldb, n0, (long)(kt + 1) * BK, N, K);
+    }
+    const T* as = OBA_AS(cur);
+    const T* bs = OBA_BS(cur);
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; kk++) {
+      const int kr = kk * 4 + (lane >> 4);
+      T fa[FM], fb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; i++) fa[i] = as[kr * LDA_S + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; j++) fb[j] = bs[kr * LDB_S + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = mf::mma(fb[j], fa[i], acc[i][j]);
+    }
+    if (more) {
+      la.template store<LDA_S>(OBA_AS(cur ^ 1));
+      lb.template store<LDB_S>(OBA_BS(cur ^ 1));
+    }
+    __syncthreads();
+  }
+
+#undef OBA_AS
+#undef OBA_BS
+  const bool bz = (beta == T(0));
+#pragma unroll
+  for (int i = 0; i < FM; i++) {
+    const long m = m0 + wm0 + i * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < FN; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long n = n0 + wn0 + j * 16 + mf::orow(lane, r);
+        if (m < M && n < N && !(tri == 1 && n > m) && !(tri == 2 && m > n)) {
+          T* p = C + m + n * ldc;
+          T v = alpha * acc[i][j][r];
+          if (!bz) v += beta * *p;
+          *p = v;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void scale_kernel(long m, long n, T beta, T* c, long ldc) {
+  long total = m * n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    long i = idx % m, j = idx / m;
+    T* p = c + i + j * ldc;
+    *p = (beta == T(0)) ? T(0) : beta * *p;
+  }
+}
+
+template <typename T>
+void scale(long m, long n, T beta, T* c, long ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0 || beta == T(1)) return;
+  long total = m * n;
+  int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+  scale_kernel<T><<<blocks, 256, 0, s>>>(m, n, beta, c, ldc);
+  OBA_HIP(hipGetLastError());
+}
+
+template <typename T, int BM, int BN, int BK, bool VEC>
+static void launch_cfg(bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
+                       const T* B, long ldb, T beta, T* C, long ldc, hipStream_t s, int tri,
+                       const int* abort_flag) {
+  long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  if (tm * tn > 0x7fffffffL) fatal("gemm: grid too large", nullptr);
+  dim3 grid((unsigned)(tm * tn)), block(256);
+#define OBA_GEMM_L(TA_, TB_) \
+  gemm_kernel<T, BM, BN, BK, TA_, TB_, VEC><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int)tm, (int)tn, tri, abort_flag)
+  if (!ta && !tb) OBA_GEMM_L(false, false);
+  else if (!ta && tb) OBA_GEMM_L(false, true);
+  else if (ta && !tb) OBA_GEMM_L(true, false);
+  else OBA_GEMM_L(true, true);
+#undef OBA_GEMM_L
+  OBA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long lda, const T* b,
+          long ldb, T beta, T* c, long ldc, hipStream_t s, int tri, const int* abort_flag) {
+  if (m <= 0 || n <= 0) return;
+  if (k <= 0 || alpha == T(0)) {
+    if (tri) fatal("gemm: triangular update with k==0 or alpha==0 is not supported", nullptr);
+    scale<T>(m, n, beta, c, ldc, s);
+    return;
+  }
+  constexpr int V = Vec16<T>::n;
+  const bool vec = (lda % V == 0) && (ldb % V == 0) && (((uintptr_t)a & 15) == 0) && (((uintptr_t)b & 15) == 0);
+  const long big_tiles = ((m + 127) / 128) * ((n + 127) / 128);
+  const bool small = big_tiles < rt().num_cu;
+  constexpr int BKB = sizeof(T) == 8 ? 16 : 32;
+  if (small) {
+    if (vec) launch_cfg<T, 64, 64, BKB, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    else launch_cfg<T, 64, 64, BKB, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+  } else {
+    if (vec) launch_cfg<T, 128, 128, BKB, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    else launch_cfg<T, 128, 128, BKB, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+  }
+}
+
+template void gemm<double>(bool, bool, long, long, long, double, const double*, long, const double*, long, double, double*, long, hipStream_t, int, const int*);
+template void gemm<float>(bool, bool, long, long, long, float, const float*, long, const float*, long, float, float*, long, hipStream_t, int, const int*);
+template void scale<double>(long, long, double, double*, long, hipStream_t);
+template void scale<float>(long, long, float, float*, long, hipStream_t);
+
+}  // namespace oba

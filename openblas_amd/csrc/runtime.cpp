@@ -1,0 +1,258 @@
+// Host runtime of libopenblas_amd: device/stream singleton, device scratch
+// pool, host<->device staging of caller matrices, xerbla plumbing, config
+// strings and the small extension API used by tests and benchmarks.
+//
+// Reference counterparts: driver/others/memory.c:1161,1328 (buffer pool),
+// driver/others/xerbla.c:56-64 (weak xerbla_), driver/others/openblas_env.c
+// (environment), interface/... thread-count knobs (openblas_set_num_threads,
+// cblas.h:13-24).  There is no CPU fallback anywhere: a missing device is a
+// fatal error.
+#include "common.h"
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <dlfcn.h>
+#include <map>
+#include <vector>
+#include <string>
+
+namespace oba {
+
+[[noreturn]] void fatal(const char* what, const char* detail) {
+  fprintf(stderr, "libopenblas_amd: fatal: %s%s%s\n", what, detail ? ": " : "", detail ? detail : "");
+  fflush(stderr);
+  abort();
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fatal(what, hipGetErrorString(e));
+}
+
+static Runtime* g_rt = nullptr;
+static std::once_flag g_rt_once;
+
+Runtime& rt() {
+  std::call_once(g_rt_once, [] {
+    Runtime* r = new Runtime();
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+      fatal("no HIP device available (libopenblas_amd has no CPU path)", e != hipSuccess ? hipGetErrorString(e) : nullptr);
+    int dev = 0;
+    if (const char* s = getenv("OPENBLAS_AMD_DEVICE")) dev = atoi(s);
+    if (dev < 0 || dev >= n) fatal("OPENBLAS_AMD_DEVICE out of range", nullptr);
+    OBA_HIP(hipSetDevice(dev));
+    r->device = dev;
+    hipDeviceProp_t prop;
+    OBA_HIP(hipGetDeviceProperties(&prop, dev));
+    r->num_cu = prop.multiProcessorCount;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fatal("libopenblas_amd is built for gfx950 only; device is", prop.gcnArchName);
+    OBA_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    OBA_HIP(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+    OBA_HIP(hipEventCreateWithFlags(&r->ev_a, hipEventDisableTiming));
+    OBA_HIP(hipEventCreateWithFlags(&r->ev_b, hipEventDisableTiming));
+    if (const char* s = getenv("OPENBLAS_AMD_ASYNC")) r->sync_on_return = (atoi(s) == 0);
+    g_rt = r;
+  });
+  OBA_HIP(hipSetDevice(g_rt->device));
+  return *g_rt;
+}
+
+void end_call() {
+  Runtime& r = rt();
+  if (r.sync_on_return) OBA_HIP(hipStreamSynchronize(r.stream));
+}
+
+// ---- device pool -----------------------------------------------------------
+namespace {
+std::mutex g_pool_mu;
+std::multimap<size_t, void*> g_free;       // size -> block
+std::map<void*, size_t> g_size;            // every live or cached block
+size_t g_cached_bytes = 0;
+
+size_t round_size(size_t b) {
+  if (b <= (1u << 20)) return (b + 255) & ~size_t(255);
+  return (b + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes) {
+  rt();
+  size_t sz = round_size(bytes ? bytes : 1);
+  {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_free.lower_bound(sz);
+    if (it != g_free.end() && it->first <= 2 * sz + (4u << 20)) {
+      void* p = it->second;
+      g_cached_bytes -= it->first;
+      g_free.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    // release the cache (after draining all queued work) and retry once
+    OBA_HIP(hipDeviceSynchronize());
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    for (auto& kv : g_free) { OBA_HIP(hipFree(kv.second)); g_size.erase(kv.second); }
+    g_free.clear();
+    g_cached_bytes = 0;
+    e = hipMalloc(&p, sz);
+    if (e != hipSuccess) fatal("device allocation failed", hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_size[p] = sz;
+  return p;
+}
+
+void pool_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  auto it = g_size.find(p);
+  if (it == g_size.end()) fatal("pool_free of unknown block", nullptr);
+  g_free.emplace(it->second, p);
+  g_cached_bytes += it->second;
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  rt();
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+template <typename T>
+void Mat<T>::in(T* p, long r, long c, long ldp, bool copy_in) {
+  rows = r; cols = c; host = nullptr; host_ld = ldp;
+  if (r <= 0 || c <= 0) { d = p; ld = ldp; return; }
+  if (is_device_ptr(p)) { d = p; ld = ldp; return; }
+  host = p;
+  ld = (r + 15) & ~15L;
+  buf = DevBuf(sizeof(T) * (size_t)ld * (size_t)c);
+  d = buf.as<T>();
+  if (copy_in)
+    OBA_HIP(hipMemcpy2DAsync(d, sizeof(T) * ld, p, sizeof(T) * ldp, sizeof(T) * r, c,
+                             hipMemcpyHostToDevice, rt().stream));
+}
+
+template <typename T>
+void Mat<T>::out() {
+  if (!host || rows <= 0 || cols <= 0) return;
+  OBA_HIP(hipMemcpy2DAsync(host, sizeof(T) * host_ld, d, sizeof(T) * ld, sizeof(T) * rows, cols,
+                           hipMemcpyDeviceToHost, rt().stream));
+  OBA_HIP(hipStreamSynchronize(rt().stream));
+}
+
+template struct Mat<double>;
+template struct Mat<float>;
+template struct Mat<int>;
+
+// ---- xerbla ----------------------------------------------------------------
+static thread_local char g_last_name[16] = {0};
+static thread_local int g_last_info = 0;
+
+typedef int (*xerbla_fn)(char*, blasint*, blasint);
+
+void report_xerbla(const char* name, blasint info) {
+  strncpy(g_last_name, name, sizeof(g_last_name) - 1);
+  g_last_info = info;
+  // Route through whichever xerbla_ wins symbol resolution in the process
+  // (a user override, as the reference's weak alias allows), else ours.
+  char buf[16];
+  strncpy(buf, name, sizeof(buf) - 1);
+  buf[sizeof(buf) - 1] = 0;
+  xerbla_fn f = (xerbla_fn)dlsym(RTLD_DEFAULT, "xerbla_");
+  blasint i = info;
+  if (f) f(buf, &i, (blasint)strlen(buf));
+  else printf(" ** On entry to %6s parameter number %2d had an illegal value\n", buf, i);
+}
+
+}  // namespace oba
+
+using namespace oba;
+
+// Weak so that applications may override it (driver/others/xerbla.c:56-64).
+extern "C" __attribute__((visibility("default"), weak)) int xerbla_(char* message, blasint* info, blasint len) {
+  (void)len;
+  printf(" ** On entry to %6s parameter number %2d had an illegal value\n", message, *info);
+  return 0;
+}
+
+// ---- informational API (cblas.h:13-27) -------------------------------------
+static char g_config[256];
+static int g_threads = 1;
+
+OBA_API char* openblas_get_config(void) {
+  snprintf(g_config, sizeof(g_config), "OpenBLAS-AMD gfx950 MI355X HIP MFMA LP64 (device-resident level-3 + LAPACK LU/Cholesky)");
+  return g_config;
+}
+OBA_API char* openblas_get_corename(void) { static char n[] = "gfx950"; return n; }
+OBA_API void openblas_set_num_threads(int n) { g_threads = n > 0 ? n : 1; }
+OBA_API int openblas_get_num_threads(void) { return g_threads; }
+OBA_API int openblas_get_num_procs(void) { return rt().num_cu; }
+
+// ---- extension API (no reference counterpart; used by tests/benchmarks) ----
+OBA_API void* openblas_amd_malloc(size_t bytes) {
+  rt();
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  return p;
+}
+OBA_API void openblas_amd_free(void* p) { if (p) { rt(); OBA_HIP(hipFree(p)); } }
+OBA_API int openblas_amd_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  Runtime& r = rt();
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  OBA_HIP(hipMemcpyAsync(dst, src, bytes, k, r.stream));
+  OBA_HIP(hipStreamSynchronize(r.stream));
+  return 0;
+}
+OBA_API int openblas_amd_memset(void* dst, int value, size_t bytes) {
+  Runtime& r = rt();
+  OBA_HIP(hipMemsetAsync(dst, value, bytes, r.stream));
+  OBA_HIP(hipStreamSynchronize(r.stream));
+  return 0;
+}
+OBA_API int openblas_amd_synchronize(void) { OBA_HIP(hipStreamSynchronize(rt().stream)); return 0; }
+OBA_API void openblas_amd_set_sync(int on) { rt().sync_on_return = on != 0; }
+OBA_API int openblas_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return n;
+}
+OBA_API int openblas_amd_device_info(char* name, int len, int* num_cu, int* clock_khz) {
+  Runtime& r = rt();
+  hipDeviceProp_t p;
+  OBA_HIP(hipGetDeviceProperties(&p, r.device));
+  if (name && len > 0) { strncpy(name, p.gcnArchName, len - 1); name[len - 1] = 0; }
+  if (num_cu) *num_cu = p.multiProcessorCount;
+  if (clock_khz) *clock_khz = p.clockRate;
+  return 0;
+}
+// HIP-event timer on the library stream: every kernel the library launches
+// goes to this stream, so start/stop bracket exactly the library's work.
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+OBA_API void openblas_amd_timer_start(void) {
+  Runtime& r = rt();
+  if (!g_t0) { OBA_HIP(hipEventCreate(&g_t0)); OBA_HIP(hipEventCreate(&g_t1)); }
+  OBA_HIP(hipEventRecord(g_t0, r.stream));
+}
+OBA_API float openblas_amd_timer_stop(void) {
+  Runtime& r = rt();
+  OBA_HIP(hipEventRecord(g_t1, r.stream));
+  OBA_HIP(hipEventSynchronize(g_t1));
+  float ms = 0.f;
+  OBA_HIP(hipEventElapsedTime(&ms, g_t0, g_t1));
+  return ms;
+}
+OBA_API int openblas_amd_last_xerbla(char* name, int len) {
+  if (name && len > 0) { strncpy(name, g_last_name, len - 1); name[len - 1] = 0; }
+  int i = g_last_info;
+  g_last_info = 0;
+  g_last_name[0] = 0;
+  return i;
+}
