@@ -35,10 +35,10 @@ template <typename T>
 void gemm_run(int transa, int transb, long m, long n, long k, T alpha, const T* a, long lda, const T* b,
               long ldb, T beta, T* c, long ldc) {
   if (m == 0 || n == 0) return;
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
   const bool only_scale = (k == 0 || alpha == T(0));
   if (only_scale && beta == T(1)) return;
+  Runtime& r = rt();
+  std::lock_guard<std::recursive_mutex> g(r.lock);
   Mat<T> A, B, C;
   if (!only_scale) {
     A.in(const_cast<T*>(a), transa ? k : m, transa ? m : k, lda, true);

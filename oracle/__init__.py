@@ -1,0 +1,152 @@
+"""CPU oracle for the level-3 hot path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes view of oracle/liboracle.so (plain C restatement of the reference's
+algorithms, see oracle.c for the file:line each function follows).  Only
+tests/, the smoke check and bench.py's CPU-baseline leg use it, always as the
+checker; the product (openblas_amd/) never imports it.
+
+Each function takes and returns numpy arrays (column-major copies are made
+as needed) so tests read like the reference's own testers.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = c_void_p
+        sig = {
+            "or_dgemm": (None, [c_int, c_int, c_int, c_int, c_int, c_double, P, c_int, P, c_int, c_double, P, c_int]),
+            "or_sgemm": (None, [c_int, c_int, c_int, c_int, c_int, c_float, P, c_int, P, c_int, c_float, P, c_int]),
+            "or_dtrsm": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, P, c_int, P, c_int]),
+            "or_strsm": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, P, c_int, P, c_int]),
+            "or_dgetf2": (c_int, [c_int, c_int, P, c_int, P]),
+            "or_sgetf2": (c_int, [c_int, c_int, P, c_int, P]),
+            "or_dlaswp": (None, [c_int, P, c_int, c_int, c_int, P, c_int]),
+            "or_slaswp": (None, [c_int, P, c_int, c_int, c_int, P, c_int]),
+            "or_dgetrs": (None, [c_int, c_int, c_int, P, c_int, P, P, c_int]),
+            "or_sgetrs": (None, [c_int, c_int, c_int, P, c_int, P, P, c_int]),
+            "or_dpotf2": (c_int, [c_int, c_int, P, c_int]),
+            "or_spotf2": (c_int, [c_int, c_int, P, c_int]),
+            "or_dsgesv": (c_int, [c_int, c_int, P, c_int, P, P, c_int, P, c_int, P]),
+            "or_dmmch_ratio": (c_double, [c_int, c_int, c_int, c_int, c_int, c_double, P, c_int, P, c_int, c_double,
+                                          P, c_int, P, c_int, c_double]),
+            "or_dbeg": (c_double, [c_int]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _f(a, dt):
+    return np.array(a, dtype=dt, order="F", copy=True)
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def gemm(transa, transb, alpha, A, B, beta, C):
+    """C <- alpha*op(A)*op(B) + beta*C (returns a new array)."""
+    dt = np.float64 if np.asarray(C).dtype == np.float64 else np.float32
+    A, B, C = _f(A, dt), _f(B, dt), _f(C, dt)
+    m, n = C.shape
+    k = A.shape[0] if transa else A.shape[1]
+    f = lib().or_dgemm if dt == np.float64 else lib().or_sgemm
+    f(int(transa), int(transb), m, n, k, alpha, _p(A), max(A.shape[0], 1), _p(B), max(B.shape[0], 1), beta, _p(C),
+      max(m, 1))
+    return C
+
+
+def trsm(side, uplo, transa, diag, alpha, A, B):
+    """side 0 left / 1 right, uplo 1 lower / 0 upper, diag 1 non-unit / 0 unit."""
+    dt = np.float64 if np.asarray(B).dtype == np.float64 else np.float32
+    A, B = _f(A, dt), _f(B, dt)
+    m, n = B.shape
+    f = lib().or_dtrsm if dt == np.float64 else lib().or_strsm
+    f(side, uplo, transa, diag, m, n, alpha, _p(A), max(A.shape[0], 1), _p(B), max(m, 1))
+    return B
+
+
+def getf2(A):
+    """Returns (LU, ipiv 1-based int32, info)."""
+    dt = np.float64 if np.asarray(A).dtype == np.float64 else np.float32
+    A = _f(A, dt)
+    m, n = A.shape
+    ipiv = np.zeros(max(min(m, n), 1), np.int32)
+    f = lib().or_dgetf2 if dt == np.float64 else lib().or_sgetf2
+    info = f(m, n, _p(A), max(m, 1), _p(ipiv))
+    return A, ipiv[:min(m, n)], info
+
+
+def laswp(A, k1, k2, ipiv, incx):
+    dt = np.float64 if np.asarray(A).dtype == np.float64 else np.float32
+    A = _f(A, dt)
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    f = lib().or_dlaswp if dt == np.float64 else lib().or_slaswp
+    f(A.shape[1], _p(A), max(A.shape[0], 1), k1, k2, _p(ipiv), incx)
+    return A
+
+
+def getrs(trans, LU, ipiv, B):
+    dt = np.float64 if np.asarray(B).dtype == np.float64 else np.float32
+    LU, B = _f(LU, dt), _f(B, dt)
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    n, nrhs = B.shape
+    f = lib().or_dgetrs if dt == np.float64 else lib().or_sgetrs
+    f(int(trans), n, nrhs, _p(LU), max(n, 1), _p(ipiv), _p(B), max(n, 1))
+    return B
+
+
+def potf2(uplo, A):
+    """uplo 1 lower / 0 upper.  Returns (factor, info)."""
+    dt = np.float64 if np.asarray(A).dtype == np.float64 else np.float32
+    A = _f(A, dt)
+    f = lib().or_dpotf2 if dt == np.float64 else lib().or_spotf2
+    info = f(uplo, A.shape[0], _p(A), max(A.shape[0], 1))
+    return A, info
+
+
+def dsgesv(A, B):
+    """Returns (A_out, ipiv, X, iter, info)."""
+    A, B = _f(A, np.float64), _f(B, np.float64)
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    ipiv = np.zeros(max(n, 1), np.int32)
+    it = c_int(0)
+    info = lib().or_dsgesv(n, nrhs, _p(A), max(n, 1), _p(ipiv), _p(B), max(n, 1), _p(X), max(n, 1),
+                           ctypes.addressof(it))
+    return A, ipiv[:n], X, it.value, info
+
+
+def dmmch_ratio(transa, transb, alpha, A, B, beta, C_in, C_out, eps=None):
+    A, B, C_in, C_out = (_f(x, np.float64) for x in (A, B, C_in, C_out))
+    m, n = C_in.shape
+    k = A.shape[0] if transa else A.shape[1]
+    if eps is None:
+        eps = np.finfo(np.float64).eps
+    return lib().or_dmmch_ratio(int(transa), int(transb), m, n, k, alpha, _p(A), max(A.shape[0], 1), _p(B),
+                                max(B.shape[0], 1), beta, _p(C_in), max(m, 1), _p(C_out), max(m, 1), eps)
+
+
+def dbeg(reset=False):
+    return lib().or_dbeg(1 if reset else 0)

@@ -1,0 +1,134 @@
+"""Writes tests/golden/level3_golden.npz: input/output vectors produced by
+the reference library itself (scipy's bundled OpenBLAS 0.3.28, called through
+its exported entry points by oracle/refblas.py) for the functions on the
+level-3 hot path.  The oracle tests check the C oracle against these, the
+GPU tests check libopenblas_amd against them.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from oracle import refblas as R  # noqa: E402
+
+
+def householder_matrix(n, kappa, rng, nref=8):
+    """Q1 diag(s) Q2, Qi = products of nref Householder reflectors with
+    Gaussian vectors, s geometric from 1 to 1/kappa (dlatms MODE 3)."""
+    s = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = np.diag(s)
+    for _ in range(nref):
+        v = rng.standard_normal(n)
+        A = A - np.outer(v, (2.0 / (v @ v)) * (v @ A))
+    for _ in range(nref):
+        v = rng.standard_normal(n)
+        A = A - np.outer((2.0 / (v @ v)) * (A @ v), v)
+    return np.asfortranarray(A)
+
+
+def main():
+    rng = np.random.default_rng(20241202)
+    out = {}
+    # --- gemm: 4 trans x shapes (odd, lda padding), alpha/beta incl. 0, NaN in C with beta 0
+    cases = [(1, 1, 1), (7, 5, 3), (35, 33, 37), (67, 17, 29)]
+    i = 0
+    for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+        for (m, n, k) in cases:
+            for ta in (0, 1):
+                for tb in (0, 1):
+                    for (alpha, beta) in ((1.0, 0.0), (0.7, 1.3), (0.0, 1.3)):
+                        pad = 3
+                        Ar, Ac = (k, m) if ta else (m, k)
+                        Br, Bc = (n, k) if tb else (k, n)
+                        A = np.asfortranarray(rng.uniform(-1, 1, (Ar + pad, Ac)).astype(dt))
+                        B = np.asfortranarray(rng.uniform(-1, 1, (Br + pad, Bc)).astype(dt))
+                        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)).astype(dt))
+                        if beta == 0.0:
+                            C[0, 0] = np.nan   # beta == 0 must not propagate NaN
+                        Cin = C.copy(order="F")
+                        Cv = np.asfortranarray(C[:m])
+                        R.cblas_gemm(ta, tb, alpha, np.asfortranarray(A[:Ar]), np.asfortranarray(B[:Br]), beta, Cv)
+                        key = f"gemm{tag}_{i}"
+                        out[key + "_meta"] = np.array([ta, tb, m, n, k, Ar + pad, Br + pad, m + pad], np.int64)
+                        out[key + "_ab"] = np.array([alpha, beta])
+                        out[key + "_A"], out[key + "_B"], out[key + "_Cin"], out[key + "_Cout"] = A, B, Cin, Cv
+                        i += 1
+    # --- trsm: all 16 side/uplo/trans/diag, two sizes
+    i = 0
+    for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+        for (m, n) in ((9, 5), (70, 33)):
+            for side in (0, 1):
+                for uplo in (0, 1):
+                    for tr in (0, 1):
+                        for diag in (0, 1):
+                            na = n if side else m
+                            A = rng.uniform(-1, 1, (na, na))
+                            A[np.diag_indices(na)] = na + rng.uniform(0, 1, na)
+                            A = np.asfortranarray(A.astype(dt))
+                            B = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+                            X = B.copy(order="F")
+                            R.cblas_trsm(side, uplo, tr, diag, 0.7, A, X)
+                            key = f"trsm{tag}_{i}"
+                            out[key + "_meta"] = np.array([side, uplo, tr, diag, m, n], np.int64)
+                            out[key + "_A"], out[key + "_B"], out[key + "_X"] = A, B, X
+                            i += 1
+    # --- getrf: random square/wide/tall, exact zero column (info), reversed magnitudes
+    mats = []
+    mats.append(rng.standard_normal((50, 50)))
+    mats.append(rng.standard_normal((37, 61)))
+    mats.append(rng.standard_normal((61, 37)))
+    Z = rng.standard_normal((40, 40)); Z[:, 10] = 0.0; mats.append(Z)
+    rev = np.diag(np.arange(1, 21, dtype=np.float64)[::-1].copy()) + 0.01 * rng.standard_normal((20, 20))
+    mats.append(rev[::-1].copy())
+    for j, M in enumerate(mats):
+        for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+            A = np.asfortranarray(M.astype(dt))
+            LU = A.copy(order="F")
+            ipiv, info = R.getrf(LU)
+            key = f"getrf{tag}_{j}"
+            out[key + "_A"], out[key + "_LU"], out[key + "_ipiv"] = A, LU, ipiv
+            out[key + "_info"] = np.array([info])
+            if A.shape[0] == A.shape[1] and info == 0:
+                for tr in (0, 1):
+                    B = np.asfortranarray(rng.standard_normal((A.shape[0], 3)).astype(dt))
+                    X = B.copy(order="F")
+                    R.getrs(tr, LU.copy(order="F"), ipiv, X)
+                    out[key + f"_rs{tr}_B"], out[key + f"_rs{tr}_X"] = B, X
+    # --- potrf: SPD (B B^T + n I), both triangles; a non-PD matrix (info)
+    for j, n in enumerate((1, 2, 40, 70)):
+        Bm = rng.standard_normal((n, n))
+        A = Bm @ Bm.T + n * np.eye(n)
+        if n == 2:
+            A = np.array([[2.0, 0.3], [0.3, 3.0]])   # utest/test_potrs.c:403 smoketest_trivial
+        for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+            for uplo in (0, 1):
+                F = np.asfortranarray(A.astype(dt))
+                info = R.potrf(uplo, F)
+                key = f"potrf{tag}_{j}_{uplo}"
+                out[key + "_A"], out[key + "_F"], out[key + "_info"] = np.asfortranarray(A.astype(dt)), F, np.array([info])
+    NP = np.eye(12); NP[7, 7] = -1.0; NP[3, 5] = NP[5, 3] = 0.2
+    for uplo in (0, 1):
+        F = np.asfortranarray(NP.copy())
+        info = R.potrf(uplo, F)
+        out[f"potrfd_np_{uplo}_A"], out[f"potrfd_np_{uplo}_F"], out[f"potrfd_np_{uplo}_info"] = NP, F, np.array([info])
+    # --- dsgesv: kappa in {1e2, 1e6, 1e10}, nrhs in {1, 2}
+    for j, kappa in enumerate((1e2, 1e6, 1e10)):
+        A = householder_matrix(64, kappa, rng)
+        for nrhs in (1, 2):
+            B = np.asfortranarray(rng.standard_normal((64, nrhs)))
+            X, ipiv, it, info = R.dsgesv(A.copy(order="F"), B.copy(order="F"))
+            key = f"dsgesv_{j}_{nrhs}"
+            out[key + "_A"], out[key + "_B"], out[key + "_X"] = A, B, X
+            out[key + "_iter"], out[key + "_info"] = np.array([it]), np.array([info])
+    out["reference_config"] = np.array(R.config())
+    path = os.path.join(HERE, "level3_golden.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()
