@@ -16,6 +16,7 @@
 // are mapped XCD-aware: the 8 XCDs each get a contiguous, grouped block of C
 // tiles so that the panels of A and B they share stay in their own L2.
 #include "kernels.h"
+#include <type_traits>
 
 namespace oba {
 
@@ -54,9 +55,13 @@ struct TileLoader {
   static_assert(NV * V * 256 == TILE * BK, "tile must divide over 256 threads");
   T r[NV * V];
 
+  // GUARD: per-element bounds checks (edge tiles, K tail); otherwise the
+  // caller guarantees the whole slab is in range and the loads are
+  // unconditional 16-byte vector loads.
+  template <bool GUARD>
   __device__ inline void load(const T* __restrict__ g, long ld, long mn0, long k0, long MN, long K) {
     const int tid = threadIdx.x;
-    const bool full = (mn0 + TILE <= MN) && (k0 + BK <= K);
+    constexpr bool full = !GUARD;
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       const int v = tid + i * 256;
@@ -161,22 +166,10 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
 #pragma unroll
     for (int j = 0; j < FN; j++) acc[i][j] = acc_t{};
 
-  const int nk = (int)((K + BK - 1) / BK);
-  la.load(A, lda, m0, 0, M, K);
-  lb.load(B, ldb, n0, 0, N, K);
-  la.template store<LDA_S>(OBA_AS(0));
-  lb.template store<LDB_S>(OBA_BS(0));
-  __syncthreads();
-
-  for (int kt = 0; kt < nk; kt++) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < nk;
-    if (more) {
-      la.load(A, lda, m0, (long)(kt + 1) * BK, M, K);
-      lb.load(B, ldb, n0, (long)(kt + 1) * BK, N, K);
-    }
-    const T* as = OBA_AS(cur);
-    const T* bs = OBA_BS(cur);
+  // compute one staged slab: BK/4 MFMA k-steps on the wave's FM x FN tiles
+  auto compute = [&](int buf) __attribute__((always_inline)) {
+    const T* as = OBA_AS(buf);
+    const T* bs = OBA_BS(buf);
 #pragma unroll
     for (int kk = 0; kk < BK / 4; kk++) {
       const int kr = kk * 4 + (lane >> 4);
@@ -190,12 +183,53 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = mf::mma(fb[j], fa[i], acc[i][j]);
     }
-    if (more) {
+  };
+  // Full slabs run with unconditional loads on interior tiles (no control
+  // flow around the prefetch, so its loads stay in flight across the MFMAs);
+  // the K tail and edge tiles use guarded loads.
+  auto run = [&](auto guard_tag) __attribute__((always_inline)) {
+    constexpr bool G = decltype(guard_tag)::value;
+    const int nkf = (int)(K / BK);
+    const bool tail = (K % BK) != 0;
+    if (nkf == 0) {
+      la.template load<true>(A, lda, m0, 0, M, K);
+      lb.template load<true>(B, ldb, n0, 0, N, K);
+      la.template store<LDA_S>(OBA_AS(0));
+      lb.template store<LDB_S>(OBA_BS(0));
+      __syncthreads();
+      compute(0);
+      return;
+    }
+    la.template load<G>(A, lda, m0, 0, M, K);
+    lb.template load<G>(B, ldb, n0, 0, N, K);
+    la.template store<LDA_S>(OBA_AS(0));
+    lb.template store<LDB_S>(OBA_BS(0));
+    __syncthreads();
+    for (int kt = 0; kt < nkf - 1; kt++) {
+      const int cur = kt & 1;
+      la.template load<G>(A, lda, m0, (long)(kt + 1) * BK, M, K);
+      lb.template load<G>(B, ldb, n0, (long)(kt + 1) * BK, N, K);
+      compute(cur);
       la.template store<LDA_S>(OBA_AS(cur ^ 1));
       lb.template store<LDB_S>(OBA_BS(cur ^ 1));
+      __syncthreads();
     }
-    __syncthreads();
-  }
+    const int last = (nkf - 1) & 1;
+    if (tail) {
+      la.template load<true>(A, lda, m0, (long)nkf * BK, M, K);
+      lb.template load<true>(B, ldb, n0, (long)nkf * BK, N, K);
+    }
+    compute(last);
+    if (tail) {
+      la.template store<LDA_S>(OBA_AS(last ^ 1));
+      lb.template store<LDB_S>(OBA_BS(last ^ 1));
+      __syncthreads();
+      compute(last ^ 1);
+    }
+  };
+  const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+  if (interior) run(std::false_type{});
+  else run(std::true_type{});
 
 #undef OBA_AS
 #undef OBA_BS

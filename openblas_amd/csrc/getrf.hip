@@ -37,17 +37,7 @@
 namespace oba {
 
 constexpr int LU_W = 32;        // leaf panel width
-constexpr int LU_RPT = 2;       // rows per thread in the panel kernel
-constexpr int LU_PMAX = 256;    // max workgroups of the panel kernel
-
-template <typename F, int... J>
-__device__ __attribute__((always_inline)) inline void static_for_impl(F&& f, std::integer_sequence<int, J...>) {
-  (f(std::integral_constant<int, J>{}), ...);
-}
-template <int N, typename F>
-__device__ __attribute__((always_inline)) inline void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
+constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
 
 template <typename T> struct MinNormal;
 template <> struct MinNormal<double> { static constexpr double v = DBL_MIN; };
@@ -91,195 +81,279 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
   if (i < n) row[i * stride] = (T)i;
 }
 
-// ---- cooperative panel kernel ----------------------------------------------
-// Factors A(j0:m, j0:j0+w) held as At(j0:j0+w, j0:m).  Workgroup g owns
-// panel rows [g*R, (g+1)*R), thread t rows g*R + q*256 + t.
-template <typename T, int W, int RPT>
-__global__ __launch_bounds__(256) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
-                                                       int* __restrict__ ipiv, int* __restrict__ info,
-                                                       unsigned* counter, unsigned base, unsigned* err,
-                                                       double* cval, int* cidx, T* crow, T* rowj) {
+// ---- panel kernel ----------------------------------------------------------
+// Factors A(j0:m, j0:j0+w) held as At(j0:j0+w, j0:m).  Workgroup g owns the
+// panel rows [g*ROWS, (g+1)*ROWS) in registers, thread t rows g*ROWS+q*NT+t.
+// Rows are never moved inside the kernel: each keeps its current position
+// (LAPACK order after the interchanges so far), so an interchange is an
+// O(1) update of two positions and the rows land at their final positions
+// when written back.  Per column, every workgroup publishes its best
+// candidate (position, row number, whole row) as 64-bit words that carry
+// 32 payload bits and a 32-bit epoch; a relaxed agent-scope 64-bit store or
+// load is single-copy atomic, so readers poll the words themselves (one
+// round trip, no fences, no counters) and all workgroups reduce the same
+// candidates to the same pivot.
+constexpr int LU_NT = 512;            // threads per workgroup
+constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
+// rows per thread: FP64 rows take twice the registers
+template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
+
+__device__ inline void put_word(unsigned long long* p, unsigned payload, unsigned epoch) {
+  __hip_atomic_store(p, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long get_word(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T> struct Words;
+template <> struct Words<float> {
+  static constexpr int n = 1;
+  __device__ static void put(unsigned long long* p, float v, unsigned ep) { put_word(p, __float_as_uint(v), ep); }
+  __device__ static float get(const unsigned* s) { return __uint_as_float(s[0]); }
+};
+template <> struct Words<double> {
+  static constexpr int n = 2;
+  __device__ static void put(unsigned long long* p, double v, unsigned ep) {
+    put_word(p, (unsigned)__double2hiint(v), ep);
+    put_word(p + 1, (unsigned)__double2loint(v), ep);
+  }
+  __device__ static double get(const unsigned* s) { return __hiloint2double((int)s[0], (int)s[1]); }
+};
+
+// a panel row lives in registers as one vector value; a dynamic element
+// index lowers to register selects instead of forcing the row to scratch
+template <typename T, int W> struct RowVec { typedef T type __attribute__((ext_vector_type(W))); };
+
+template <typename V, int W>
+__device__ __attribute__((always_inline)) inline auto pick(const V& x, int j) {
+  auto v = x[0];
+#pragma unroll
+  for (int c = 1; c < W; c++) v = (c == j) ? x[c] : v;
+  return v;
+}
+
+// pivot key: |x|; a NaN only wins when it sits at the current position j
+// (IAMAX returns the first element when that one is NaN, and otherwise never
+// selects a NaN -- kernel/arm/iamax.c:51-74)
+template <typename T>
+__device__ inline double pivot_key(T xv, int pos, int j) {
+  return isnan(xv) ? (pos == j ? (double)INFINITY : -1.0) : fabs((double)xv);
+}
+
+__device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
+
+template <typename T, int W, int RPT, int NT>
+__global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
+                                                      int* __restrict__ ipiv, int* __restrict__ info,
+                                                      int* __restrict__ moves, unsigned long long* rec,
+                                                      unsigned epoch0, unsigned* err) {
+  constexpr int NWV = Words<T>::n;
+  constexpr int RW = 2 + W * NWV;                 // words per record
+  constexpr int ROWS = NT * RPT;
+  constexpr int NWAVE = NT / 64;
+  constexpr int NPT = (LU_PMAX * RW + NT - 1) / NT;
   const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const long mp = m - j0;
-  constexpr long R = 256L * RPT;
-  __shared__ double s_val[4];
-  __shared__ int s_idx[4];
-  __shared__ int s_best, s_win, s_dead;
-  __shared__ T u[W], rj[W];
+  __shared__ unsigned srec[LU_PMAX * RW];
+  __shared__ T u[W];
+  __shared__ double s_v[NWAVE];
+  __shared__ int s_p[NWAVE], s_o[NWAVE];
+  __shared__ int s_win, s_bad, s_p2p[W];
 
-  T x[RPT][W];
+  typename RowVec<T, W>::type x[RPT];
+  int pos[RPT];
+  bool live[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
-    const long lr = g * R + q * 256 + t;
-    const T* src = At + j0 + (j0 + lr) * ldt;
+    const long r = (long)g * ROWS + q * NT + t;
+    live[q] = r < mp;
+    pos[q] = (int)r;
+    const T* src = At + j0 + (j0 + r) * ldt;
 #pragma unroll
-    for (int c = 0; c < W; c++) x[q][c] = (lr < mp && c < w) ? src[c] : T(0);
+    for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
   }
-  if (t == 0) s_dead = 0;
+  if (t < W) s_p2p[t] = t;
+  if (t == 0) s_bad = 0;
   __syncthreads();
 
-  bool dead = false;
-  // the column loop is expanded at compile time so that x[][] stays in VGPRs
-  static_for<W>([&](auto jc) __attribute__((always_inline)) {
-    constexpr int j = decltype(jc)::value;
-    if (j < w && !dead) {
+  for (int j = 0; j < w; j++) {
+    const unsigned ep = epoch0 + (unsigned)j + 1u;
     const int buf = j & 1;
-    // (a) local candidate: max |x|, lowest row on ties, NaN never wins
-    double bv = -1.0;
-    int bi = INT_MAX;
+    unsigned long long* rbuf = rec + (size_t)buf * LU_PMAX * RW;
+    // (a) this workgroup's best candidate: max key, lowest position on ties
+    double bv = -2.0;
+    int bp = INT_MAX, bo = -1;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      const long lr = g * R + q * 256 + t;
-      if (lr >= j && lr < mp) {
-        double v = fabs((double)x[q][j]);
-        if (v > bv) { bv = v; bi = (int)lr; }
+      if (live[q]) {
+        double v = pivot_key((T)pick<decltype(x[q]), W>(x[q], j), pos[q], j);
+        if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; bo = t * RPT + q; }
       }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       double ov = __shfl_xor(bv, off);
-      int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      int op = __shfl_xor(bp, off), oo = __shfl_xor(bo, off);
+      if (better(ov, op, bv, bp)) { bv = ov; bp = op; bo = oo; }
     }
-    if ((t & 63) == 0) { s_val[t >> 6] = bv; s_idx[t >> 6] = bi; }
+    if ((t & 63) == 0) { s_v[t >> 6] = bv; s_p[t >> 6] = bp; s_o[t >> 6] = bo; }
     __syncthreads();
-    if (t == 0) {
-      double v = s_val[0];
-      int id = s_idx[0];
-      for (int k = 1; k < 4; k++)
-        if (s_val[k] > v || (s_val[k] == v && s_idx[k] < id)) { v = s_val[k]; id = s_idx[k]; }
-      s_best = id;
-      cval[buf * LU_PMAX + g] = v;
-      cidx[buf * LU_PMAX + g] = id;
-    }
-    __syncthreads();
-    // (b) publish the candidate row and the current row j
-    {
-      const int best = s_best;
+    double wv_ = s_v[0];
+    int wp = s_p[0], wo = s_o[0];
+#pragma unroll
+    for (int k = 1; k < NWAVE; k++)
+      if (better(s_v[k], s_p[k], wv_, wp)) { wv_ = s_v[k]; wp = s_p[k]; wo = s_o[k]; }
+    // (b) publish: the owning thread writes the record (no candidate: thread 0
+    //     writes an empty one so that every word carries the epoch)
+    unsigned long long* myrec = rbuf + (size_t)g * RW;
+    if (wo >= 0 && t == wo / RPT) {
+      const int qq = wo % RPT;
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
-        const long lr = g * R + q * 256 + t;
-        if (lr == best) {
+        if (q == qq) {
+          put_word(myrec, (unsigned)pos[q], ep);
+          put_word(myrec + 1, (unsigned)((long)g * ROWS + q * NT + t), ep);
 #pragma unroll
-          for (int c = 0; c < W; c++)
-            if (c < w) crow[((long)buf * LU_PMAX + g) * W + c] = x[q][c];
-        }
-        if (lr == j) {
-#pragma unroll
-          for (int c = 0; c < W; c++)
-            if (c < w) rowj[buf * W + c] = x[q][c];
+          for (int c = 0; c < W; c++) Words<T>::put(myrec + 2 + c * NWV, x[q][c], ep);
         }
       }
+    } else if (wo < 0 && t == 0) {
+      put_word(myrec, (unsigned)INT_MAX, ep);
+      put_word(myrec + 1, 0u, ep);
+      for (int c = 0; c < W * NWV; c++) put_word(myrec + 2 + c, 0u, ep);
     }
-    __syncthreads();
-    // (c) grid barrier (bounded spin; a timeout is reported, never hangs)
-    if (t == 0 && !s_dead) {
-      __threadfence();
-      atomicAdd(counter, 1u);
-      const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
-      long spins = 0;
-      while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > (1L << 22)) { atomicExch(err, 1u); s_dead = 1; break; }
-      }
-      __threadfence();
-    }
-    __syncthreads();
-    dead = s_dead != 0;
-    if (!dead) {
-    // (d) global pivot: reduce the P candidates
+    // (c) gather all P records: issue every load, then re-poll stale words
     {
-      double gv = -1.0;
-      int gi = INT_MAX;
-      if (t < P) { gv = cval[buf * LU_PMAX + t]; gi = cidx[buf * LU_PMAX + t]; }
+      const int total = P * RW;
+      unsigned long long wv[NPT];
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int idx = t + k * NT;
+        if (idx < total) wv[k] = get_word(rbuf + idx);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < NPT; k++) {
+        const int idx = t + k * NT;
+        if (idx < total) {
+          long spins = 0;
+          while ((unsigned)(wv[k] >> 32) != ep) {
+            if (++spins > (1L << 21)) { bad = true; break; }
+            wv[k] = get_word(rbuf + idx);
+          }
+          srec[idx] = (unsigned)wv[k];
+        }
+      }
+      if (bad) { s_bad = 1; atomicExch(err, 1u); }
+    }
+    __syncthreads();
+    if (s_bad) break;
+    // (d) the pivot: one wavefront reduces the P candidates
+    if (t < 64) {
+      double v = -2.0;
+      int p = INT_MAX, wi = -1;
+      for (int c = t; c < P; c += 64) {
+        const int cp = (int)srec[c * RW];
+        if (cp == INT_MAX) continue;
+        const double vv = pivot_key(Words<T>::get(&srec[c * RW + 2 + j * NWV]), cp, j);
+        if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
+      }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(gv, off);
-        int oi = __shfl_xor(gi, off);
-        if (ov > gv || (ov == gv && oi < gi)) { gv = ov; gi = oi; }
+        double ov = __shfl_xor(v, off);
+        int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
+        if (better(ov, op, v, p)) { v = ov; p = op; wi = ow; }
       }
-      if ((t & 63) == 0) { s_val[t >> 6] = gv; s_idx[t >> 6] = gi; }
-      __syncthreads();
-      if (t == 0) {
-        double v = s_val[0];
-        int id = s_idx[0];
-        for (int k = 1; k < 4; k++)
-          if (s_val[k] > v || (s_val[k] == v && s_idx[k] < id)) { v = s_val[k]; id = s_idx[k]; }
-        s_win = id;
-      }
-      __syncthreads();
-    }
-    const int win = s_win;
-    const int p = (win == INT_MAX) ? j : win;   // all-NaN column: keep row j
-    if (t < w) {
-      rj[t] = rowj[buf * W + t];
-      u[t] = (win == INT_MAX) ? rj[t] : crow[((long)buf * LU_PMAX + p / R) * W + t];
+      if (t == 0) s_win = wi;
     }
     __syncthreads();
+    const int wi = s_win;
+    if (wi < 0 || wi >= P) { s_bad = 1; atomicExch(err, 2u); break; }   // cannot happen: mp > j rows are live
+    if (t < W) u[t] = Words<T>::get(&srec[wi * RW + 2 + t * NWV]);
+    __syncthreads();
+    const int q_win = (int)srec[wi * RW];
+    const int phys_win = (int)srec[wi * RW + 1];
+    if (q_win < j || q_win >= mp || phys_win < 0 || phys_win >= mp) { s_bad = 1; atomicExch(err, 2u); break; }
     const T piv = u[j];
     const bool nz = piv != T(0);
     const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
+    const int a = s_p2p[j];
     if (g == 0 && t == 0) {
-      ipiv[j0 + j] = (int)(j0 + p + 1);
+      ipiv[j0 + j] = (int)(j0 + q_win + 1);
       if (!nz && *info == 0) *info = (int)(j0 + j + 1);
     }
     const T rinv = doscale ? T(1) / piv : T(1);
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      const long lr = g * R + q * 256 + t;
-      if (lr < mp) {
-        if (p != j) {
-          if (lr == j) {
+      const int r = g * ROWS + q * NT + t;
+      if (r == phys_win) { live[q] = false; pos[q] = j; }
+      else if (r == a) pos[q] = q_win;
+      if (live[q]) {
+        T l = pick<decltype(x[q]), W>(x[q], j);
+        if (doscale) l *= rinv;
 #pragma unroll
-            for (int c = 0; c < W; c++)
-              if (c < w && (c > j || doscale)) x[q][c] = u[c];
-          } else if (lr == p) {
-#pragma unroll
-            for (int c = 0; c < W; c++)
-              if (c < w && (c > j || doscale)) x[q][c] = rj[c];
-          }
-        }
-        if (lr > j) {
-          T l = x[q][j];
-          if (doscale) { l *= rinv; x[q][j] = l; }
-#pragma unroll
-          for (int c = j + 1; c < W; c++)
-            if (c < w) x[q][c] -= l * u[c];
+        for (int c = 0; c < W; c++) {
+          if (c == j) x[q][c] = l;
+          else if (c > j && c < w) x[q][c] -= l * u[c];
         }
       }
     }
     __syncthreads();
-    }  // !dead
-    }  // j < w
-  });
-
+    if (t == 0) {
+      s_p2p[j] = phys_win;
+      if (q_win < W) s_p2p[q_win] = a;
+    }
+  }
+  if (s_bad) return;
+  // every row to its final position; list the rows that moved
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
-    const long lr = g * R + q * 256 + t;
-    if (lr < mp) {
-      T* dst = At + j0 + (j0 + lr) * ldt;
+    const long r = (long)g * ROWS + q * NT + t;
+    if (r < mp) {
+      T* dst = At + j0 + (j0 + pos[q]) * ldt;
 #pragma unroll
       for (int c = 0; c < W; c++)
         if (c < w) dst[c] = x[q][c];
+      if (pos[q] != (int)r) {
+        const int k = atomicAdd(moves, 1);
+        if (k < 2 * W) {   // at most 2w rows change position
+          moves[1 + 2 * k] = pos[q];
+          moves[2 + 2 * k] = (int)r;
+        }
+      }
     }
   }
 }
 
-// Apply the interchanges of one panel (rows j0..j0+w-1 of A) to every other
-// column of A, i.e. to rows [0, nrows_t) \ [j0, j0+w) of At.
-template <typename T>
-__global__ void lu_swap_kernel(T* __restrict__ At, long ldt, long nrows_t, long j0, int w, const int* __restrict__ ipiv) {
+// Apply the panel's interchanges to every other column of A, i.e. to the
+// rows [0, nrows_t) \ [j0, j0+w) of At: the moves (destination position,
+// source row) form a permutation of at most 2w columns of At, so every
+// source element is read before any destination is written (no sequential
+// dependence, coalesced along the rows of At).
+template <typename T, int W>
+__global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long ldt, long nrows_t, long j0, int w,
+                                                      long mp, const int* __restrict__ moves) {
+  __shared__ int sd[2 * W], ss[2 * W];
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = min(max(moves[0], 0), 2 * W);
+  __syncthreads();
+  const int cnt = s_cnt;
+  if ((int)threadIdx.x < cnt) {
+    int d = moves[1 + 2 * threadIdx.x], sr = moves[2 + 2 * threadIdx.x];
+    if (d < 0 || d >= mp || sr < 0 || sr >= mp) d = sr = 0;   // never index outside the panel rows
+    sd[threadIdx.x] = d;
+    ss[threadIdx.x] = sr;
+  }
+  __syncthreads();
   const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (r >= nrows_t || (r >= j0 && r < j0 + w)) return;
-  for (int k = 0; k < w; k++) {
-    const long a = j0 + k, p = ipiv[j0 + k] - 1;
-    if (p != a) {
-      T* x = At + r + a * ldt;
-      T* y = At + r + p * ldt;
-      T tmp = *x;
-      *x = *y;
-      *y = tmp;
-    }
-  }
+  T v[2 * W];
+#pragma unroll
+  for (int k = 0; k < 2 * W; k++)
+    if (k < cnt) v[k] = At[r + (j0 + ss[k]) * ldt];
+#pragma unroll
+  for (int k = 0; k < 2 * W; k++)
+    if (k < cnt) At[r + (j0 + sd[k]) * ldt] = v[k];
 }
 
 template <typename T>
@@ -288,31 +362,29 @@ struct LuCtx {
   long ldt, m, nrows_t;
   int* ipiv;
   int* info;
-  unsigned* counter;
+  int* moves;          // LU_MV ints per panel: a count and up to 2*LU_W pairs
+  long panel_no = 0;
+  unsigned long long* rec;
+  unsigned epoch = 0;
   unsigned* err;
-  unsigned base = 0;
-  double* cval;
-  int* cidx;
-  T* crow;
-  T* rowj;
   hipStream_t s;
-  int max_coop_blocks;
 };
 
 template <typename T>
 static void lu_panel(LuCtx<T>& c, long j0, int w) {
   const long mp = c.m - j0;
-  const long R = 256L * LU_RPT;
-  long P = (mp + R - 1) / R;
-  if (P > LU_PMAX || P > c.max_coop_blocks) fatal("getrf: too many rows for the panel kernel", nullptr);
-  void* args[] = {&c.At, &c.ldt, &c.m, &j0, &w, &c.ipiv, &c.info, &c.counter, &c.base, &c.err,
-                  &c.cval, &c.cidx, &c.crow, &c.rowj};
-  OBA_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T, LU_W, LU_RPT>, dim3((unsigned)P), dim3(256),
-                                     args, 0, c.s));
-  c.base += (unsigned)P * (unsigned)w;
-  long others = c.nrows_t - w;
-  if (others > 0) {
-    lu_swap_kernel<T><<<(unsigned)((c.nrows_t + 255) / 256), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, c.ipiv);
+  constexpr int RPT = LuRpt<T>::v;
+  constexpr long ROWS = (long)LU_NT * RPT;
+  long P = (mp + ROWS - 1) / ROWS;
+  if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
+  int* mv = c.moves + c.panel_no * LU_MV;
+  lu_panel_kernel<T, LU_W, RPT, LU_NT><<<(unsigned)P, LU_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
+                                                                        c.rec, c.epoch, c.err);
+  OBA_HIP(hipGetLastError());
+  c.epoch += (unsigned)w;
+  c.panel_no++;
+  if (c.nrows_t - w > 0) {
+    lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 255) / 256), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);
     OBA_HIP(hipGetLastError());
   }
 }
@@ -341,32 +413,30 @@ template <typename T>
 int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hipStream_t s) {
   const long mn = std::min(m, n);
   if (mn <= 0) return 0;
-  DevBuf ws(4096 + sizeof(double) * 2 * LU_PMAX + sizeof(int) * 2 * LU_PMAX + sizeof(T) * (2 * LU_PMAX * LU_W + 2 * LU_W));
+  constexpr int RW = 2 + LU_W * Words<T>::n;
+  const long npanels = (mn + LU_W - 1) / LU_W + 1;
+  const size_t rec_bytes = sizeof(unsigned long long) * 2 * LU_PMAX * RW;
+  const size_t mv_bytes = sizeof(int) * npanels * LU_MV;
+  DevBuf ws(4096 + rec_bytes + mv_bytes);
   char* p = ws.as<char>();
   LuCtx<T> c;
   c.At = At; c.ldt = ldt; c.m = m; c.nrows_t = n + (track_perm ? 1 : 0); c.ipiv = ipiv_dev; c.s = s;
   c.info = (int*)p;
-  c.counter = (unsigned*)(p + 64);
-  c.err = (unsigned*)(p + 128);
-  c.cval = (double*)(p + 4096);
-  c.cidx = (int*)(c.cval + 2 * LU_PMAX);
-  c.crow = (T*)(c.cidx + 2 * LU_PMAX);
-  c.rowj = c.crow + 2 * LU_PMAX * LU_W;
-  OBA_HIP(hipMemsetAsync(p, 0, 4096, s));
-  int per_cu = 0;
-  OBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_kernel<T, LU_W, LU_RPT>, 256, 0));
-  c.max_coop_blocks = per_cu * rt().num_cu;
+  c.err = (unsigned*)(p + 64);
+  c.rec = (unsigned long long*)(p + 4096);
+  c.moves = (int*)(p + 4096 + rec_bytes);
+  OBA_HIP(hipMemsetAsync(p, 0, 4096 + rec_bytes + mv_bytes, s));
   if (track_perm) {
     iota_row_kernel<T><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(At + n, m, ldt);
     OBA_HIP(hipGetLastError());
   }
   lu_rec<T>(c, 0, mn);
   if (n > mn) trsm<T>(true, false, false, true, n - mn, mn, T(1), At, ldt, At + mn, ldt, s);
-  int h[3] = {0, 0, 0};
+  int h[2] = {0, 0};
   OBA_HIP(hipMemcpyAsync(&h[0], c.info, sizeof(int), hipMemcpyDeviceToHost, s));
   OBA_HIP(hipMemcpyAsync(&h[1], c.err, sizeof(int), hipMemcpyDeviceToHost, s));
   OBA_HIP(hipStreamSynchronize(s));
-  if (h[1]) fatal("getrf: panel grid barrier timed out", nullptr);
+  if (h[1]) fatal("getrf: panel candidate exchange timed out", nullptr);
   return h[0];
 }
 

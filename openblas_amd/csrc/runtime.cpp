@@ -55,7 +55,12 @@ Runtime& rt() {
     if (const char* s = getenv("OPENBLAS_AMD_ASYNC")) r->sync_on_return = (atoi(s) == 0);
     g_rt = r;
   });
-  OBA_HIP(hipSetDevice(g_rt->device));
+  // make the library's device current once per host thread
+  static thread_local bool t_dev_set = false;
+  if (!t_dev_set) {
+    OBA_HIP(hipSetDevice(g_rt->device));
+    t_dev_set = true;
+  }
   return *g_rt;
 }
 

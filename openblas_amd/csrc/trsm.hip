@@ -31,44 +31,53 @@ __device__ inline float bcast_lane(float v, int k) {
 // Solves E x = b for nvec vectors, E = op(A) (left) or op(A)^T (right),
 // nb <= 64.  Vector v, element i lives at B[i*inc_e + v*inc_v].
 // eT: E(i,k) = A(k,i) (else A(i,k)); elower: E is lower triangular.
+constexpr int TRSM_VPW = 4;   // right-hand-side vectors per wavefront
+
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
                                                         T* __restrict__ B, long inc_e, long inc_v, bool eT,
                                                         bool elower, bool unit, const int* abort_flag) {
-  __shared__ T E[TRSM_NB * TRSM_NB];   // E(i,k) at E[k*64 + i]
+  constexpr int LDE = TRSM_NB + 1;     // padded: conflict-free transposed stores
+  __shared__ T E[TRSM_NB * LDE];       // E(i,k) at E[k*LDE + i]
   __shared__ T dinv[TRSM_NB];
   if (abort_flag && *abort_flag) return;
+  // consecutive threads read consecutive addresses of A in both layouts
   for (int idx = threadIdx.x; idx < TRSM_NB * TRSM_NB; idx += blockDim.x) {
-    int i = idx & 63, k = idx >> 6;
+    const int f = idx & 63, s = idx >> 6;       // f: fast (contiguous) index of A
+    // eT: E(i,k) = A(k,i) -> A[f + s*lda] = A(f, s) = E(s, f)
+    const int i = eT ? s : f, k = eT ? f : s;
     T v = T(0);
-    if (i < nb && k < nb) v = eT ? A[k + (long)i * lda] : A[i + (long)k * lda];
-    E[k * TRSM_NB + i] = v;
+    if (f < nb && s < nb) v = A[f + (long)s * lda];
+    E[k * LDE + i] = v;
   }
   __syncthreads();
   if (threadIdx.x < TRSM_NB) {
-    int k = threadIdx.x;
-    dinv[k] = (k < nb && !unit) ? T(1) / E[k * TRSM_NB + k] : T(1);
+    const int k = threadIdx.x;
+    dinv[k] = (k < nb && !unit) ? T(1) / E[k * LDE + k] : T(1);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const long v = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (v >= nvec) return;
-  T* b = B + v * inc_v;
-  T x = lane < nb ? b[lane * inc_e] : T(0);
-  if (elower) {
-    for (int k = 0; k < nb; k++) {
-      if (lane == k) x *= dinv[k];
-      T xk = bcast_lane(x, k);
-      if (lane > k) x -= E[k * TRSM_NB + lane] * xk;
+  const long v0 = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * TRSM_VPW;
+  for (int vv = 0; vv < TRSM_VPW; vv++) {
+    const long v = v0 + vv;
+    if (v >= nvec) return;
+    T* b = B + v * inc_v;
+    T x = lane < nb ? b[lane * inc_e] : T(0);
+    if (elower) {
+      for (int k = 0; k < nb; k++) {
+        if (lane == k) x *= dinv[k];
+        T xk = bcast_lane(x, k);
+        if (lane > k) x -= E[k * LDE + lane] * xk;
+      }
+    } else {
+      for (int k = nb - 1; k >= 0; k--) {
+        if (lane == k) x *= dinv[k];
+        T xk = bcast_lane(x, k);
+        if (lane < k) x -= E[k * LDE + lane] * xk;
+      }
     }
-  } else {
-    for (int k = nb - 1; k >= 0; k--) {
-      if (lane == k) x *= dinv[k];
-      T xk = bcast_lane(x, k);
-      if (lane < k) x -= E[k * TRSM_NB + lane] * xk;
-    }
+    if (lane < nb) b[lane * inc_e] = x;
   }
-  if (lane < nb) b[lane * inc_e] = x;
 }
 
 template <typename T>
@@ -81,7 +90,8 @@ static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nr
   const long inc_e = right ? ldb : 1;
   const long inc_v = right ? 1 : ldb;
   const int waves = 4;
-  long blocks = (nrhs + waves - 1) / waves;
+  const long per_block = (long)waves * TRSM_VPW;
+  long blocks = (nrhs + per_block - 1) / per_block;
   trsm_leaf_kernel<T><<<(unsigned)blocks, 64 * waves, 0, s>>>((int)nb, nrhs, a, lda, b, inc_e, inc_v, eT, elower,
                                                               unit, abort_flag);
   OBA_HIP(hipGetLastError());

@@ -215,3 +215,39 @@ def test_dsgesv_live_reference_kappa2():
     _, _, X, it, info = oracle.dsgesv(A, B)
     assert it == itr >= 0 and info == infor == 0
     assert np.max(np.abs(X - Xr)) < 1e-13
+
+
+def test_ctest_grid_harness_on_reference_library():
+    """The din3 grid harness used by the GPU test, run against the reference
+    library itself: pins the row-/column-major argument mapping of the
+    harness and the dmmch checker (ctest/c_dblat3c.c:2956) on real data."""
+    import ctypes
+    L = refblas.lib()
+    f = L.scipy_cblas_dgemm if hasattr(L, "scipy_cblas_dgemm") else None
+    if f is None:
+        pytest.skip("installed reference library is the ILP64 build")
+    I, D, P = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    TR = {0: 111, 1: 112}
+    sizes = [1, 2, 3, 7, 35]
+    worst = 0.0
+    oracle.dbeg(True)
+    for (m, n, k) in [(m, n, k) for m in sizes for n in sizes for k in sizes if (m + n + k) % 3 == 0]:
+        for ta in (0, 1):
+            for tb in (0, 1):
+                Ar, Ac = (k, m) if ta else (m, k)
+                Br, Bc = (n, k) if tb else (k, n)
+                A = np.array([[oracle.dbeg() for _ in range(Ac)] for _ in range(Ar + 1)], order="F")
+                B = np.array([[oracle.dbeg() for _ in range(Bc)] for _ in range(Br + 1)], order="F")
+                C0 = np.array([[oracle.dbeg() for _ in range(n)] for _ in range(m + 1)], order="F")
+                for alpha, beta in ((0.0, 1.3), (1.0, 0.0), (0.7, 1.3)):
+                    C = C0.copy(order="F")
+                    f(I(102), I(TR[ta]), I(TR[tb]), I(m), I(n), I(k), D(alpha), P(A.ctypes.data), I(Ar + 1),
+                      P(B.ctypes.data), I(Br + 1), D(beta), P(C.ctypes.data), I(m + 1))
+                    worst = max(worst, oracle.dmmch_ratio(ta, tb, alpha, A[:Ar], B[:Br], beta, C0[:m], C[:m]))
+                    Cr = np.array(C0[:m], order="C", copy=True)
+                    Ac_ = np.array(A[:Ar], order="C", copy=True)
+                    Bc_ = np.array(B[:Br], order="C", copy=True)
+                    f(I(101), I(TR[ta]), I(TR[tb]), I(m), I(n), I(k), D(alpha), P(Ac_.ctypes.data), I(Ac),
+                      P(Bc_.ctypes.data), I(Bc), D(beta), P(Cr.ctypes.data), I(n))
+                    worst = max(worst, oracle.dmmch_ratio(ta, tb, alpha, A[:Ar], B[:Br], beta, C0[:m], Cr))
+    assert worst < 16
