@@ -222,29 +222,38 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       put_word(myrec + 1, 0u, ep);
       for (int c = 0; c < W * NWV; c++) put_word(myrec + 2 + c, 0u, ep);
     }
-    // (c) gather all P records: issue every load, then re-poll stale words
+    // (c) gather all P records: issue every load, then re-poll all stale
+    //     words together (one round trip per poll, never one per word)
     {
       const int total = P * RW;
       unsigned long long wv[NPT];
+      bool need[NPT];
 #pragma unroll
       for (int k = 0; k < NPT; k++) {
         const int idx = t + k * NT;
-        if (idx < total) wv[k] = get_word(rbuf + idx);
+        need[k] = idx < total;
+        if (need[k]) wv[k] = get_word(rbuf + idx);
       }
-      bool bad = false;
+      long spins = 0;
+      for (;;) {
+        bool stale = false;
 #pragma unroll
-      for (int k = 0; k < NPT; k++) {
-        const int idx = t + k * NT;
-        if (idx < total) {
-          long spins = 0;
-          while ((unsigned)(wv[k] >> 32) != ep) {
-            if (++spins > (1L << 21)) { bad = true; break; }
-            wv[k] = get_word(rbuf + idx);
+        for (int k = 0; k < NPT; k++) {
+          if (need[k]) {
+            if ((unsigned)(wv[k] >> 32) == ep) {
+              srec[t + k * NT] = (unsigned)wv[k];
+              need[k] = false;
+            } else {
+              stale = true;
+            }
           }
-          srec[idx] = (unsigned)wv[k];
         }
+        if (!stale) break;
+        if (++spins > (1L << 21)) { s_bad = 1; atomicExch(err, 1u); break; }
+#pragma unroll
+        for (int k = 0; k < NPT; k++)
+          if (need[k]) wv[k] = get_word(rbuf + t + k * NT);
       }
-      if (bad) { s_bad = 1; atomicExch(err, 1u); }
     }
     __syncthreads();
     if (s_bad) break;

@@ -80,6 +80,58 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const
   }
 }
 
+// Right-side leaves: the vectors are rows of B (contiguous across vectors,
+// stride ldb along a vector), so one lane owns one whole vector in
+// registers and the loads/stores of a wavefront are coalesced; the triangle
+// is read from LDS as broadcasts.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_leaf_rows_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
+                                                             T* __restrict__ B, long ldb, bool eT, bool elower,
+                                                             bool unit, const int* abort_flag) {
+  constexpr int LDE = TRSM_NB + 1;
+  __shared__ T Er[TRSM_NB * LDE];      // E(c,k) at Er[c*LDE + k]
+  __shared__ T dinv[TRSM_NB];
+  if (abort_flag && *abort_flag) return;
+  for (int idx = threadIdx.x; idx < TRSM_NB * TRSM_NB; idx += blockDim.x) {
+    const int f = idx & 63, s = idx >> 6;   // A(f, s), f contiguous
+    const int c = eT ? s : f, k = eT ? f : s;
+    T v = T(0);
+    if (f < nb && s < nb) v = A[f + (long)s * lda];
+    Er[c * LDE + k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < TRSM_NB) {
+    const int k = threadIdx.x;
+    dinv[k] = (k < nb && !unit) ? T(1) / Er[k * LDE + k] : T(1);
+  }
+  __syncthreads();
+  const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (v >= nvec) return;
+  T x[TRSM_NB];
+#pragma unroll
+  for (int c = 0; c < TRSM_NB; c++) x[c] = (c < nb) ? B[v + c * ldb] : T(0);
+  if (elower) {
+#pragma unroll
+    for (int c = 0; c < TRSM_NB; c++) {
+      T acc = x[c];
+#pragma unroll
+      for (int k = 0; k < c; k++) acc -= Er[c * LDE + k] * x[k];
+      x[c] = acc * dinv[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = TRSM_NB - 1; c >= 0; c--) {
+      T acc = x[c];
+#pragma unroll
+      for (int k = c + 1; k < TRSM_NB; k++) acc -= Er[c * LDE + k] * x[k];
+      x[c] = acc * dinv[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TRSM_NB; c++)
+    if (c < nb) B[v + c * ldb] = x[c];
+}
+
 template <typename T>
 static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nrhs, const T* a, long lda, T* b,
                  long ldb, hipStream_t s, const int* abort_flag) {
@@ -87,6 +139,12 @@ static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nr
   // E = op(A) for left, op(A)^T for right
   const bool eT = trans != right;
   const bool elower = lower != eT;
+  if (right) {
+    trsm_leaf_rows_kernel<T><<<(unsigned)((nrhs + 255) / 256), 256, 0, s>>>((int)nb, nrhs, a, lda, b, ldb, eT, elower,
+                                                                           unit, abort_flag);
+    OBA_HIP(hipGetLastError());
+    return;
+  }
   const long inc_e = right ? ldb : 1;
   const long inc_v = right ? 1 : ldb;
   const int waves = 4;
