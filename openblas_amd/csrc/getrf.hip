@@ -88,40 +88,19 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // (LAPACK order after the interchanges so far), so an interchange is an
 // O(1) update of two positions and the rows land at their final positions
 // when written back.  Per column, every workgroup publishes its best
-// candidate (position, row number, whole row) as 64-bit words that carry
-// 32 payload bits and a 32-bit epoch; a relaxed agent-scope 64-bit store or
-// load is single-copy atomic, so readers poll the words themselves (one
-// round trip, no fences, no counters) and all workgroups reduce the same
-// candidates to the same pivot.
+// candidate (position, row number, whole row); one grid barrier (release
+// fence, atomic arrival counter, acquire fence -- the cheapest exchange
+// measured on MI355X) makes all candidates visible, and every workgroup
+// reduces the same candidates to the same pivot, so no second exchange is
+// needed for the pivot row.
 constexpr int LU_NT = 512;            // threads per workgroup
 constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
+
 // rows per thread: FP64 rows take twice the registers
 template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
 
-__device__ inline void put_word(unsigned long long* p, unsigned payload, unsigned epoch) {
-  __hip_atomic_store(p, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline unsigned long long get_word(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <typename T> struct Words;
-template <> struct Words<float> {
-  static constexpr int n = 1;
-  __device__ static void put(unsigned long long* p, float v, unsigned ep) { put_word(p, __float_as_uint(v), ep); }
-  __device__ static float get(const unsigned* s) { return __uint_as_float(s[0]); }
-};
-template <> struct Words<double> {
-  static constexpr int n = 2;
-  __device__ static void put(unsigned long long* p, double v, unsigned ep) {
-    put_word(p, (unsigned)__double2hiint(v), ep);
-    put_word(p + 1, (unsigned)__double2loint(v), ep);
-  }
-  __device__ static double get(const unsigned* s) { return __hiloint2double((int)s[0], (int)s[1]); }
-};
-
-// a panel row lives in registers as one vector value; a dynamic element
-// index lowers to register selects instead of forcing the row to scratch
+// a panel row lives in registers as one vector value; dynamic element
+// indices go through static selects so the row never spills to scratch
 template <typename T, int W> struct RowVec { typedef T type __attribute__((ext_vector_type(W))); };
 
 template <typename V, int W>
@@ -142,19 +121,27 @@ __device__ inline double pivot_key(T xv, int pos, int j) {
 
 __device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
 
+// candidate records of one column: [2 buffers][LU_PMAX] positions, row
+// numbers and rows (double-buffered by column parity: a workgroup can only
+// publish column j+2 after every workgroup has read column j)
+template <typename T>
+struct PanelRec {
+  int* pos;
+  int* phys;
+  T* row;
+};
+
 template <typename T, int W, int RPT, int NT>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
                                                       int* __restrict__ ipiv, int* __restrict__ info,
-                                                      int* __restrict__ moves, unsigned long long* rec,
-                                                      unsigned epoch0, unsigned* err) {
-  constexpr int NWV = Words<T>::n;
-  constexpr int RW = 2 + W * NWV;                 // words per record
+                                                      int* __restrict__ moves, PanelRec<T> rec, unsigned* counter,
+                                                      unsigned base, unsigned* err) {
   constexpr int ROWS = NT * RPT;
   constexpr int NWAVE = NT / 64;
-  constexpr int NPT = (LU_PMAX * RW + NT - 1) / NT;
   const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const long mp = m - j0;
-  __shared__ unsigned srec[LU_PMAX * RW];
+  __shared__ T srow[LU_PMAX * W];
+  __shared__ int spos[LU_PMAX], sphys[LU_PMAX];
   __shared__ T u[W];
   __shared__ double s_v[NWAVE];
   __shared__ int s_p[NWAVE], s_o[NWAVE];
@@ -177,9 +164,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   __syncthreads();
 
   for (int j = 0; j < w; j++) {
-    const unsigned ep = epoch0 + (unsigned)j + 1u;
     const int buf = j & 1;
-    unsigned long long* rbuf = rec + (size_t)buf * LU_PMAX * RW;
     // (a) this workgroup's best candidate: max key, lowest position on ties
     double bv = -2.0;
     int bp = INT_MAX, bo = -1;
@@ -203,68 +188,48 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll
     for (int k = 1; k < NWAVE; k++)
       if (better(s_v[k], s_p[k], wv_, wp)) { wv_ = s_v[k]; wp = s_p[k]; wo = s_o[k]; }
-    // (b) publish: the owning thread writes the record (no candidate: thread 0
-    //     writes an empty one so that every word carries the epoch)
-    unsigned long long* myrec = rbuf + (size_t)g * RW;
+    // (b) publish this workgroup's candidate (plain stores)
+    const int slot = buf * LU_PMAX + g;
     if (wo >= 0 && t == wo / RPT) {
       const int qq = wo % RPT;
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
         if (q == qq) {
-          put_word(myrec, (unsigned)pos[q], ep);
-          put_word(myrec + 1, (unsigned)((long)g * ROWS + q * NT + t), ep);
+          rec.pos[slot] = pos[q];
+          rec.phys[slot] = (int)((long)g * ROWS + q * NT + t);
 #pragma unroll
-          for (int c = 0; c < W; c++) Words<T>::put(myrec + 2 + c * NWV, x[q][c], ep);
+          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q][c];
         }
       }
     } else if (wo < 0 && t == 0) {
-      put_word(myrec, (unsigned)INT_MAX, ep);
-      put_word(myrec + 1, 0u, ep);
-      for (int c = 0; c < W * NWV; c++) put_word(myrec + 2 + c, 0u, ep);
+      rec.pos[slot] = INT_MAX;
+      rec.phys[slot] = 0;
     }
-    // (c) gather all P records: issue every load, then re-poll all stale
-    //     words together (one round trip per poll, never one per word)
-    {
-      const int total = P * RW;
-      unsigned long long wv[NPT];
-      bool need[NPT];
-#pragma unroll
-      for (int k = 0; k < NPT; k++) {
-        const int idx = t + k * NT;
-        need[k] = idx < total;
-        if (need[k]) wv[k] = get_word(rbuf + idx);
-      }
+    __syncthreads();
+    // (c) grid barrier: release, arrive, bounded spin, acquire
+    if (t == 0) {
+      __threadfence();
+      atomicAdd(counter, 1u);
+      const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
       long spins = 0;
-      for (;;) {
-        bool stale = false;
-#pragma unroll
-        for (int k = 0; k < NPT; k++) {
-          if (need[k]) {
-            if ((unsigned)(wv[k] >> 32) == ep) {
-              srec[t + k * NT] = (unsigned)wv[k];
-              need[k] = false;
-            } else {
-              stale = true;
-            }
-          }
-        }
-        if (!stale) break;
-        if (++spins > (1L << 21)) { s_bad = 1; atomicExch(err, 1u); break; }
-#pragma unroll
-        for (int k = 0; k < NPT; k++)
-          if (need[k]) wv[k] = get_word(rbuf + t + k * NT);
+      while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++spins > (1L << 22)) { s_bad = 1; atomicExch(err, 1u); break; }
       }
+      __threadfence();
     }
     __syncthreads();
     if (s_bad) break;
-    // (d) the pivot: one wavefront reduces the P candidates
+    // (d) every workgroup reads all candidates and reduces them identically
+    for (int idx = t; idx < P * W; idx += NT) srow[idx] = rec.row[(size_t)buf * LU_PMAX * W + idx];
+    if (t < P) { spos[t] = rec.pos[buf * LU_PMAX + t]; sphys[t] = rec.phys[buf * LU_PMAX + t]; }
+    __syncthreads();
     if (t < 64) {
       double v = -2.0;
       int p = INT_MAX, wi = -1;
       for (int c = t; c < P; c += 64) {
-        const int cp = (int)srec[c * RW];
+        const int cp = spos[c];
         if (cp == INT_MAX) continue;
-        const double vv = pivot_key(Words<T>::get(&srec[c * RW + 2 + j * NWV]), cp, j);
+        const double vv = pivot_key(srow[c * W + j], cp, j);
         if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
       }
 #pragma unroll
@@ -278,12 +243,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     __syncthreads();
     const int wi = s_win;
     if (wi < 0 || wi >= P) { s_bad = 1; atomicExch(err, 2u); break; }   // cannot happen: mp > j rows are live
-    if (t < W) u[t] = Words<T>::get(&srec[wi * RW + 2 + t * NWV]);
-    __syncthreads();
-    const int q_win = (int)srec[wi * RW];
-    const int phys_win = (int)srec[wi * RW + 1];
+    const int q_win = spos[wi];
+    const int phys_win = sphys[wi];
     if (q_win < j || q_win >= mp || phys_win < 0 || phys_win >= mp) { s_bad = 1; atomicExch(err, 2u); break; }
-    const T piv = u[j];
+    const T* uw = srow + wi * W;
+    const T piv = uw[j];
     const bool nz = piv != T(0);
     const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
     const int a = s_p2p[j];
@@ -303,7 +267,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll
         for (int c = 0; c < W; c++) {
           if (c == j) x[q][c] = l;
-          else if (c > j && c < w) x[q][c] -= l * u[c];
+          else if (c > j && c < w) x[q][c] -= l * uw[c];
         }
       }
     }
@@ -342,8 +306,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 template <typename T, int W>
 __global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long ldt, long nrows_t, long j0, int w,
                                                       long mp, const int* __restrict__ moves) {
+  constexpr int RB = 64;                       // At rows per workgroup
   __shared__ int sd[2 * W], ss[2 * W];
   __shared__ int s_cnt;
+  __shared__ T vals[2 * W][RB];
   if (threadIdx.x == 0) s_cnt = min(max(moves[0], 0), 2 * W);
   __syncthreads();
   const int cnt = s_cnt;
@@ -354,15 +320,20 @@ __global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long l
     ss[threadIdx.x] = sr;
   }
   __syncthreads();
-  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (r >= nrows_t || (r >= j0 && r < j0 + w)) return;
-  T v[2 * W];
-#pragma unroll
-  for (int k = 0; k < 2 * W; k++)
-    if (k < cnt) v[k] = At[r + (j0 + ss[k]) * ldt];
-#pragma unroll
-  for (int k = 0; k < 2 * W; k++)
-    if (k < cnt) At[r + (j0 + sd[k]) * ldt] = v[k];
+  const long r0 = (long)blockIdx.x * RB;
+  // gather every source first (coalesced along the rows of At) ...
+  for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
+    const int k = idx / RB, rr = idx % RB;
+    const long r = r0 + rr;
+    if (r < nrows_t && !(r >= j0 && r < j0 + w)) vals[k][rr] = At[r + (j0 + ss[k]) * ldt];
+  }
+  __syncthreads();
+  // ... then scatter: the moves are a permutation of the touched columns
+  for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
+    const int k = idx / RB, rr = idx % RB;
+    const long r = r0 + rr;
+    if (r < nrows_t && !(r >= j0 && r < j0 + w)) At[r + (j0 + sd[k]) * ldt] = vals[k][rr];
+  }
 }
 
 template <typename T>
@@ -373,8 +344,9 @@ struct LuCtx {
   int* info;
   int* moves;          // LU_MV ints per panel: a count and up to 2*LU_W pairs
   long panel_no = 0;
-  unsigned long long* rec;
-  unsigned epoch = 0;
+  PanelRec<T> rec;
+  unsigned* counter;
+  unsigned base = 0;
   unsigned* err;
   hipStream_t s;
 };
@@ -388,12 +360,12 @@ static void lu_panel(LuCtx<T>& c, long j0, int w) {
   if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
   int* mv = c.moves + c.panel_no * LU_MV;
   lu_panel_kernel<T, LU_W, RPT, LU_NT><<<(unsigned)P, LU_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
-                                                                        c.rec, c.epoch, c.err);
+                                                                        c.rec, c.counter, c.base, c.err);
   OBA_HIP(hipGetLastError());
-  c.epoch += (unsigned)w;
+  c.base += (unsigned)P * (unsigned)w;
   c.panel_no++;
   if (c.nrows_t - w > 0) {
-    lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 255) / 256), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);
+    lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 63) / 64), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);
     OBA_HIP(hipGetLastError());
   }
 }
@@ -422,9 +394,8 @@ template <typename T>
 int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hipStream_t s) {
   const long mn = std::min(m, n);
   if (mn <= 0) return 0;
-  constexpr int RW = 2 + LU_W * Words<T>::n;
   const long npanels = (mn + LU_W - 1) / LU_W + 1;
-  const size_t rec_bytes = sizeof(unsigned long long) * 2 * LU_PMAX * RW;
+  const size_t rec_bytes = (sizeof(int) * 2 + sizeof(T) * LU_W) * 2 * LU_PMAX;
   const size_t mv_bytes = sizeof(int) * npanels * LU_MV;
   DevBuf ws(4096 + rec_bytes + mv_bytes);
   char* p = ws.as<char>();
@@ -432,7 +403,10 @@ int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hip
   c.At = At; c.ldt = ldt; c.m = m; c.nrows_t = n + (track_perm ? 1 : 0); c.ipiv = ipiv_dev; c.s = s;
   c.info = (int*)p;
   c.err = (unsigned*)(p + 64);
-  c.rec = (unsigned long long*)(p + 4096);
+  c.counter = (unsigned*)(p + 128);
+  c.rec.row = (T*)(p + 4096);
+  c.rec.pos = (int*)(p + 4096 + sizeof(T) * LU_W * 2 * LU_PMAX);
+  c.rec.phys = c.rec.pos + 2 * LU_PMAX;
   c.moves = (int*)(p + 4096 + rec_bytes);
   OBA_HIP(hipMemsetAsync(p, 0, 4096 + rec_bytes + mv_bytes, s));
   if (track_perm) {

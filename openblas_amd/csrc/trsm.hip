@@ -88,8 +88,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void trsm_leaf_rows_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
                                                              T* __restrict__ B, long ldb, bool eT, bool elower,
                                                              bool unit, const int* abort_flag) {
-  constexpr int LDE = TRSM_NB + 1;
-  __shared__ T Er[TRSM_NB * LDE];      // E(c,k) at Er[c*LDE + k]
+  constexpr int LDE = TRSM_NB + 2;
+  __shared__ T Ec[TRSM_NB * LDE];      // E(c,k) at Ec[k*LDE + c]: column k contiguous
   __shared__ T dinv[TRSM_NB];
   if (abort_flag && *abort_flag) return;
   for (int idx = threadIdx.x; idx < TRSM_NB * TRSM_NB; idx += blockDim.x) {
@@ -97,12 +97,12 @@ __global__ __launch_bounds__(256) void trsm_leaf_rows_kernel(int nb, long nvec, 
     const int c = eT ? s : f, k = eT ? f : s;
     T v = T(0);
     if (f < nb && s < nb) v = A[f + (long)s * lda];
-    Er[c * LDE + k] = v;
+    Ec[k * LDE + c] = v;
   }
   __syncthreads();
   if (threadIdx.x < TRSM_NB) {
     const int k = threadIdx.x;
-    dinv[k] = (k < nb && !unit) ? T(1) / Er[k * LDE + k] : T(1);
+    dinv[k] = (k < nb && !unit) ? T(1) / Ec[k * LDE + k] : T(1);
   }
   __syncthreads();
   const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -110,21 +110,21 @@ __global__ __launch_bounds__(256) void trsm_leaf_rows_kernel(int nb, long nvec, 
   T x[TRSM_NB];
 #pragma unroll
   for (int c = 0; c < TRSM_NB; c++) x[c] = (c < nb) ? B[v + c * ldb] : T(0);
+  // column-oriented substitution: x_k is final, then an independent axpy
+  // over the remaining entries (ILP instead of a dependent dot product)
   if (elower) {
 #pragma unroll
-    for (int c = 0; c < TRSM_NB; c++) {
-      T acc = x[c];
+    for (int k = 0; k < TRSM_NB; k++) {
+      x[k] *= dinv[k];
 #pragma unroll
-      for (int k = 0; k < c; k++) acc -= Er[c * LDE + k] * x[k];
-      x[c] = acc * dinv[c];
+      for (int c = k + 1; c < TRSM_NB; c++) x[c] -= Ec[k * LDE + c] * x[k];
     }
   } else {
 #pragma unroll
-    for (int c = TRSM_NB - 1; c >= 0; c--) {
-      T acc = x[c];
+    for (int k = TRSM_NB - 1; k >= 0; k--) {
+      x[k] *= dinv[k];
 #pragma unroll
-      for (int k = c + 1; k < TRSM_NB; k++) acc -= Er[c * LDE + k] * x[k];
-      x[c] = acc * dinv[c];
+      for (int c = 0; c < k; c++) x[c] -= Ec[k * LDE + c] * x[k];
     }
   }
 #pragma unroll
@@ -140,8 +140,8 @@ static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nr
   const bool eT = trans != right;
   const bool elower = lower != eT;
   if (right) {
-    trsm_leaf_rows_kernel<T><<<(unsigned)((nrhs + 255) / 256), 256, 0, s>>>((int)nb, nrhs, a, lda, b, ldb, eT, elower,
-                                                                           unit, abort_flag);
+    trsm_leaf_rows_kernel<T><<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>((int)nb, nrhs, a, lda, b, ldb, eT, elower,
+                                                                        unit, abort_flag);
     OBA_HIP(hipGetLastError());
     return;
   }
