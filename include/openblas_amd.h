@@ -152,6 +152,13 @@ int openblas_get_num_threads(void);
 int openblas_get_num_procs(void);
 
 /* ---- extensions: no counterpart in the reference ---------------------------- */
+/* FP64 GEMM emulated with INT8 slices on the INT8 MFMA (Ozaki-style): the
+   cblas_dgemm contract plus the slice count (0 = chosen from the operands'
+   exponent spread, else 1..12).  Returns the slice count used, -1 after an
+   argument error (xerbla "DGEMM ", info 15 for a bad slice count). */
+int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                             blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
+                             const double *B, blasint ldb, double beta, double *C, blasint ldc, int slices);
 /* device memory and synchronisation for callers that keep matrices in HBM */
 void *openblas_amd_malloc(size_t bytes);
 void openblas_amd_free(void *p);

@@ -78,6 +78,8 @@ _SIGS = {
     "openblas_set_num_threads": (None, [c_int]),
     "openblas_get_num_threads": (c_int, []),
     "openblas_get_num_procs": (c_int, []),
+    "openblas_amd_dgemm_ozaki": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int,
+                                         c_double, _pd, c_int, c_int]),
     "openblas_amd_malloc": (c_void_p, [c_size_t]),
     "openblas_amd_free": (None, [c_void_p]),
     "openblas_amd_memcpy": (c_int, [c_void_p, c_void_p, c_size_t, c_int]),

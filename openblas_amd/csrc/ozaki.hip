@@ -1,0 +1,330 @@
+// FP64 GEMM emulated with INT8 slices on the INT8 MFMA (Ozaki-style
+// splitting): no counterpart in the reference; it computes the same
+// C <- alpha*op(A)*op(B) + beta*C as cblas_dgemm (interface/gemm.c:294).
+//
+// Each row i of op(A) is scaled by 2^-e_i (e_i from its largest magnitude)
+// and cut into s signed 7-bit digits a_i^(t) (t = 0..s-1, truncation, so
+// every digit lies in [-127, 127]); every column j of op(B) likewise with
+// 2^-f_j and digits b_j^(u).  Then
+//   C_ij = 2^(e_i+f_j) * sum_L 2^(-7(L+2)) * sum_{t+u=L} sum_k a_ik^(t) b_kj^(u)
+// The inner sums are INT8 GEMMs with exact int32 accumulation
+// (v_mfma_i32_16x16x64_i8); all pairs of one level L = t+u share an int32
+// accumulator (exact while (L+1)*K*127^2 < 2^31, K is chunked otherwise),
+// levels L >= s are dropped (the truncation error of the scheme), and the
+// levels are combined in FP64 from the least significant up.
+//
+// Kernels: per-vector statistics (max |x| and sum |x|, coalesced in either
+// layout), the split into k-contiguous int8 slices (64x64 tiles through
+// LDS), and one fused GEMM kernel: a workgroup owns a 64x64 tile of C, four
+// wavefronts own 32x32 each, all s slices of a 64-deep k slab of both
+// operands sit in LDS, every level's int32 accumulators and the FP64
+// accumulator stay in registers, and C is read and written once.
+#include "kernels.h"
+#include "ozaki.h"
+#include <cmath>
+#include <vector>
+
+namespace oba {
+
+typedef int i4v __attribute__((ext_vector_type(4)));
+constexpr int OZ_SMAX = 12;     // slices supported by the fused kernel
+constexpr int OZ_BK = 64;       // k per slab (one i8 MFMA k-depth)
+constexpr int OZ_T = 64;        // C tile edge
+
+// element (v, k) of an operand "vector" v at X[v*sv + k*sk]
+__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* vmax, double* vsum) {
+  const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;   // sv == 1
+  if (v >= V) return;
+  double mx = 0.0, sm = 0.0;
+  for (long k = 0; k < K; k++) {
+    double a = fabs(X[v + k * sk]);
+    mx = a > mx ? a : mx;
+    sm += a;
+  }
+  vmax[v] = mx;
+  vsum[v] = sm;
+}
+
+__global__ void oz_stats_kcontig(long V, long K, const double* __restrict__ X, long sv, double* vmax, double* vsum) {
+  const long v = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6);   // sk == 1, one wave per vector
+  const int lane = threadIdx.x & 63;
+  if (v >= V) return;
+  double mx = 0.0, sm = 0.0;
+  for (long k = lane; k < K; k += 64) {
+    double a = fabs(X[v * sv + k]);
+    mx = a > mx ? a : mx;
+    sm += a;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, off));
+    sm += __shfl_xor(sm, off);
+  }
+  if (lane == 0) { vmax[v] = mx; vsum[v] = sm; }
+}
+
+// exponent e with max*2^-e < 1 (max = f*2^e, f in [0.5,1)); 0 for max == 0
+__device__ inline int oz_exp(double mx) {
+  int e = 0;
+  if (mx > 0.0) frexp(mx, &e);
+  return e;
+}
+
+// Splits a 64(v) x 64(k) tile into s slices S[t][v][k] (k contiguous, leading
+// dimension Kp, Vp vectors per slice), zero outside V x K.
+__global__ __launch_bounds__(256) void oz_split_kernel(long V, long K, const double* __restrict__ X, long sv, long sk,
+                                                       const double* __restrict__ vmax, int s,
+                                                       signed char* __restrict__ S, long Vp, long Kp) {
+  __shared__ double tile[64][65];
+  const long v0 = (long)blockIdx.x * 64, k0 = (long)blockIdx.y * 64;
+  const bool vfast = (sv == 1);
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int f = idx & 63, g = idx >> 6;   // f: along the contiguous index
+    const int vv = vfast ? f : g, kk = vfast ? g : f;
+    const long v = v0 + vv, k = k0 + kk;
+    tile[vv][kk] = (v < V && k < K) ? X[v * sv + k * sk] : 0.0;
+  }
+  __syncthreads();
+  // each thread: one vector row of the tile, 16 consecutive k -> s x 16 bytes
+  const int vv = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+  const long v = v0 + vv;
+  const int e = (v < V) ? oz_exp(vmax[v]) : 0;
+  signed char d[OZ_SMAX][16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    double r = ldexp(tile[vv][kq + c], -e);
+#pragma unroll
+    for (int t = 0; t < OZ_SMAX; t++) {
+      r *= 128.0;
+      const double q = trunc(r);
+      r -= q;
+      d[t][c] = (signed char)(int)q;
+    }
+  }
+  for (int t = 0; t < s; t++) {
+    int4 w;
+    signed char* wb = (signed char*)&w;
+#pragma unroll
+    for (int c = 0; c < 16; c++) wb[c] = d[t][c];
+    *(int4*)(S + ((size_t)t * Vp + v) * Kp + k0 + kq) = w;
+  }
+}
+
+// The fused slice GEMM.  LDS holds all s slices of a 64-deep slab for the
+// tile's 64 rows of op(A) and 64 columns of op(B), rows padded to 80 bytes.
+template <int NS>
+__global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, long Kp, const signed char* __restrict__ SA,
+                                                      long Mp, const signed char* __restrict__ SB, long Np,
+                                                      const double* __restrict__ amax, const double* __restrict__ bmax,
+                                                      double alpha, double beta, double* __restrict__ C, long ldc,
+                                                      long kchunk) {
+  constexpr int RS = 80;   // padded LDS row stride in bytes
+  __shared__ __attribute__((aligned(16))) signed char la[NS][OZ_T * RS];
+  __shared__ __attribute__((aligned(16))) signed char lb[NS][OZ_T * RS];
+  const long tiles_m = (M + OZ_T - 1) / OZ_T;
+  const long tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+  const long m0 = tm * OZ_T, n0 = tn * OZ_T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+
+  i4v acc[NS][2][2];
+  double dacc[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) dacc[a][b][r] = 0.0;
+
+  for (long kc0 = 0; kc0 < K; kc0 += kchunk) {
+    const long kc1 = min(K, kc0 + kchunk);
+#pragma unroll
+    for (int L = 0; L < NS; L++)
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[L][a][b] = i4v{0, 0, 0, 0};
+    for (long k0 = kc0; k0 < kc1; k0 += OZ_BK) {
+      // stage the slab: NS slices x 64 rows x 64 bytes for each operand
+      for (int idx = threadIdx.x; idx < NS * OZ_T * 4; idx += 256) {
+        const int t = idx / (OZ_T * 4), rem = idx % (OZ_T * 4), row = rem >> 2, q = (rem & 3) * 16;
+        *(int4*)(&la[t][row * RS + q]) = *(const int4*)(SA + ((size_t)t * Mp + m0 + row) * Kp + k0 + q);
+        *(int4*)(&lb[t][row * RS + q]) = *(const int4*)(SB + ((size_t)t * Np + n0 + row) * Kp + k0 + q);
+      }
+      __syncthreads();
+      // fragments: lane l holds row (l%16) of a 16-row block, k bytes (l/16)*16..+15
+      i4v fb[NS][2];
+#pragma unroll
+      for (int u = 0; u < NS; u++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) fb[u][b] = *(const i4v*)(&lb[u][(wn + b * 16 + (lane & 15)) * RS + (lane >> 4) * 16]);
+#pragma unroll
+      for (int t = 0; t < NS; t++) {
+        i4v fa[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) fa[a] = *(const i4v*)(&la[t][(wm + a * 16 + (lane & 15)) * RS + (lane >> 4) * 16]);
+#pragma unroll
+        for (int u = 0; u < NS - t; u++)
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+              acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fb[u][b], fa[a], acc[t + u][a][b], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+    // fold the exact int32 levels into FP64, least significant first
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          double sum = 0.0;
+#pragma unroll
+          for (int L = NS - 1; L >= 0; L--) sum += ldexp((double)acc[L][a][b][r], -7 * (L + 2));
+          dacc[a][b][r] += sum;
+        }
+  }
+  // D[j][i]: lane l, element r -> column j = 4*(l/16)+r of the 16-block, row i = l%16
+  const bool bz = beta == 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const long i = m0 + wm + a * 16 + (lane & 15);
+    if (i >= M) continue;
+    const int ei = oz_exp(amax[i]);
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long j = n0 + wn + b * 16 + 4 * (lane >> 4) + r;
+        if (j >= N) continue;
+        const int fj = oz_exp(bmax[j]);
+        double v = alpha * ldexp(dacc[a][b][r], ei + fj);
+        double* p = C + i + j * ldc;
+        if (!bz) v += beta * *p;
+        *p = v;
+      }
+  }
+}
+
+static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vsum, hipStream_t s) {
+  if (V <= 0) return;
+  if (sv == 1) {
+    oz_stats_vcontig<<<(unsigned)((V + 255) / 256), 256, 0, s>>>(V, K, X, sk, vmax, vsum);
+  } else {
+    oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vsum);
+  }
+  OBA_HIP(hipGetLastError());
+}
+
+int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* asum, const double* bmax,
+                      const double* bsum) {
+  // peakedness of rows/columns: log2(max / mean |x|); each bit of it costs
+  // accuracy of the per-vector scaled digits of the small entries
+  auto peak = [K](long V, const double* mx, const double* sm) {
+    double worst = 0.0;
+    for (long v = 0; v < V; v++) {
+      if (mx[v] <= 0.0 || sm[v] <= 0.0) continue;
+      double r = std::log2(mx[v] * (double)K / sm[v]);
+      if (r > worst) worst = r;
+    }
+    return worst;
+  };
+  const double bits = 53.0 + peak(M, amax, asum) + peak(N, bmax, bsum);
+  int s = (int)std::ceil(bits / 7.0);
+  return std::max(2, std::min(OZ_SMAX, s));
+}
+
+int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const double* A, long lda, const double* B,
+                long ldb, double beta, double* C, long ldc, int slices, hipStream_t s) {
+  if (M <= 0 || N <= 0) return 0;
+  if (K <= 0 || alpha == 0.0) { scale<double>(M, N, beta, C, ldc, s); return 0; }
+  // op(A) row i, element k: A[i + k*lda] (N) or A[k + i*lda] (T)
+  const long a_sv = ta ? lda : 1, a_sk = ta ? 1 : lda;
+  // op(B) column j, element k: B[k + j*ldb] (N) or B[j + k*ldb] (T)
+  const long b_sv = tb ? 1 : ldb, b_sk = tb ? ldb : 1;
+  DevBuf st(sizeof(double) * 2 * (M + N));
+  double* amax = st.as<double>();
+  double* asum = amax + M;
+  double* bmax = asum + M;
+  double* bsum = bmax + N;
+  oz_stats(M, K, A, a_sv, a_sk, amax, asum, s);
+  oz_stats(N, K, B, b_sv, b_sk, bmax, bsum, s);
+  int ns = slices;
+  if (ns <= 0) {
+    std::vector<double> h(2 * (M + N));
+    OBA_HIP(hipMemcpyAsync(h.data(), amax, sizeof(double) * 2 * (M + N), hipMemcpyDeviceToHost, s));
+    OBA_HIP(hipStreamSynchronize(s));
+    ns = ozaki_slices_auto(M, N, K, h.data(), h.data() + M, h.data() + 2 * M, h.data() + 2 * M + N);
+  }
+  if (ns < 1 || ns > OZ_SMAX) fatal("ozaki: slice count out of range", nullptr);
+  const long Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
+  const long Mp = (M + OZ_T - 1) / OZ_T * OZ_T, Np = (N + OZ_T - 1) / OZ_T * OZ_T;
+  DevBuf sa((size_t)ns * Mp * Kp), sb((size_t)ns * Np * Kp);
+  {
+    dim3 ga((unsigned)(Mp / 64), (unsigned)(Kp / 64)), gb((unsigned)(Np / 64), (unsigned)(Kp / 64));
+    oz_split_kernel<<<ga, 256, 0, s>>>(M, K, A, a_sv, a_sk, amax, ns, sa.as<signed char>(), Mp, Kp);
+    oz_split_kernel<<<gb, 256, 0, s>>>(N, K, B, b_sv, b_sk, bmax, ns, sb.as<signed char>(), Np, Kp);
+    OBA_HIP(hipGetLastError());
+  }
+  // exact int32 accumulation: (pairs per level) * kchunk * 127^2 < 2^31
+  long kchunk = (long)(2147483647.0 / ((double)ns * 127.0 * 127.0)) / OZ_BK * OZ_BK;
+  if (kchunk < OZ_BK) kchunk = OZ_BK;
+  const long tiles = (Mp / OZ_T) * (Np / OZ_T);
+  if (tiles > 0x7fffffffL) fatal("ozaki: grid too large", nullptr);
+#define OZ_L(NSV)                                                                                                  \
+  case NSV:                                                                                                        \
+    oz_gemm_kernel<NSV><<<(unsigned)tiles, 256, 0, s>>>(M, N, K, Kp, sa.as<signed char>(), Mp, sb.as<signed char>(), \
+                                                        Np, amax, bmax, alpha, beta, C, ldc, kchunk);             \
+    break;
+  switch (ns) {
+    OZ_L(1) OZ_L(2) OZ_L(3) OZ_L(4) OZ_L(5) OZ_L(6) OZ_L(7) OZ_L(8) OZ_L(9) OZ_L(10) OZ_L(11) OZ_L(12)
+  }
+#undef OZ_L
+  OBA_HIP(hipGetLastError());
+  return ns;
+}
+
+}  // namespace oba
+
+using namespace oba;
+
+// Extension entry point (no reference counterpart): cblas_dgemm semantics
+// plus the slice count (0 = chosen from the operands' exponent spread).
+// Returns the slice count used, or -1 after an argument error.
+OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, blasint m, blasint n, blasint k, double alpha,
+                                     const double* a, blasint lda, const double* b, blasint ldb, double beta,
+                                     double* c, blasint ldc, int slices) {
+  int transa = -1, transb = -1;
+  long M = 0, N = 0, K = k, LDA = 0, LDB = 0;
+  const double *pa = nullptr, *pb = nullptr;
+  auto tr = [](int t) { return (t == 111 || t == 114) ? 0 : (t == 112 || t == 113) ? 1 : -1; };
+  blasint info = 0;
+  if (order == 102) { M = m; N = n; pa = a; pb = b; LDA = lda; LDB = ldb; transa = tr(TransA); transb = tr(TransB); }
+  else if (order == 101) { M = n; N = m; pa = b; pb = a; LDA = ldb; LDB = lda; transa = tr(TransB); transb = tr(TransA); }
+  if (order == 101 || order == 102) {
+    long nrowa = transa == 1 ? K : M, nrowb = transb == 1 ? N : K;
+    info = -1;
+    if (ldc < M) info = 13;
+    if (LDB < nrowb) info = 10;
+    if (LDA < nrowa) info = 8;
+    if (K < 0) info = 5;
+    if (N < 0) info = 4;
+    if (M < 0) info = 3;
+    if (transb < 0) info = 2;
+    if (transa < 0) info = 1;
+  }
+  if (info >= 0) { report_xerbla("DGEMM ", info); return -1; }
+  if (slices < 0 || slices > OZ_SMAX) { report_xerbla("DGEMM ", 15); return -1; }
+  if (M == 0 || N == 0) return 0;
+  Runtime& r = rt();
+  std::lock_guard<std::recursive_mutex> g(r.lock);
+  Mat<double> A, B, C;
+  A.in(const_cast<double*>(pa), transa ? K : M, transa ? M : K, LDA, true);
+  B.in(const_cast<double*>(pb), transb ? N : K, transb ? K : N, LDB, true);
+  C.in(c, M, N, ldc, beta != 0.0);
+  int ns = ozaki_dgemm(transa == 1, transb == 1, M, N, K, alpha, A.d, A.ld, B.d, B.ld, beta, C.d, C.ld, slices, r.stream);
+  C.out();
+  end_call();
+  return ns;
+}
