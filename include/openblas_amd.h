@@ -159,6 +159,31 @@ int openblas_get_num_procs(void);
 int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
                              blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
                              const double *B, blasint ldb, double beta, double *C, blasint ldc, int slices);
+/* Batched LU / LU solve / Cholesky of many independent small problems in one
+   persistent launch per call (the reference has no batched LAPACK; these
+   are the batched forms of interface/lapack/getrf.c:53, getrs.c:63 and
+   potrf.c:63).  Problem i has the arguments of the single routine at index
+   i of each array (matrices host or device, per pointer); info is a host
+   array receiving each problem's info exactly as the single routine would
+   set it (a problem with an illegal argument gets -position and is
+   skipped; the first one is reported through xerbla).  Returns 0, or -k
+   when argument k of the batched call itself is illegal (trans/uplo,
+   count < 0).  Problems up to 512 rows run in the batched kernels, larger
+   ones through the single-problem path. */
+int openblas_amd_dgetrf_batch(const blasint *m, const blasint *n, double *const *a, const blasint *lda,
+                              blasint *const *ipiv, blasint *info, blasint count);
+int openblas_amd_sgetrf_batch(const blasint *m, const blasint *n, float *const *a, const blasint *lda,
+                              blasint *const *ipiv, blasint *info, blasint count);
+int openblas_amd_dgetrs_batch(char trans, const blasint *n, const blasint *nrhs, double *const *a,
+                              const blasint *lda, blasint *const *ipiv, double *const *b, const blasint *ldb,
+                              blasint *info, blasint count);
+int openblas_amd_sgetrs_batch(char trans, const blasint *n, const blasint *nrhs, float *const *a,
+                              const blasint *lda, blasint *const *ipiv, float *const *b, const blasint *ldb,
+                              blasint *info, blasint count);
+int openblas_amd_dpotrf_batch(char uplo, const blasint *n, double *const *a, const blasint *lda, blasint *info,
+                              blasint count);
+int openblas_amd_spotrf_batch(char uplo, const blasint *n, float *const *a, const blasint *lda, blasint *info,
+                              blasint count);
 /* device memory and synchronisation for callers that keep matrices in HBM */
 void *openblas_amd_malloc(size_t bytes);
 void openblas_amd_free(void *p);

@@ -80,6 +80,14 @@ _SIGS = {
     "openblas_get_num_procs": (c_int, []),
     "openblas_amd_dgemm_ozaki": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int,
                                          c_double, _pd, c_int, c_int]),
+    "openblas_amd_dgetrf_batch": (c_int, [_pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p), _pi, c_int]),
+    "openblas_amd_sgetrf_batch": (c_int, [_pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p), _pi, c_int]),
+    "openblas_amd_dgetrs_batch": (c_int, [c_char, _pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p),
+                                          POINTER(c_void_p), _pi, _pi, c_int]),
+    "openblas_amd_sgetrs_batch": (c_int, [c_char, _pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p),
+                                          POINTER(c_void_p), _pi, _pi, c_int]),
+    "openblas_amd_dpotrf_batch": (c_int, [c_char, _pi, POINTER(c_void_p), _pi, _pi, c_int]),
+    "openblas_amd_spotrf_batch": (c_int, [c_char, _pi, POINTER(c_void_p), _pi, _pi, c_int]),
     "openblas_amd_malloc": (c_void_p, [c_size_t]),
     "openblas_amd_free": (None, [c_void_p]),
     "openblas_amd_memcpy": (c_int, [c_void_p, c_void_p, c_size_t, c_int]),

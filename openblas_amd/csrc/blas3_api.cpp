@@ -8,6 +8,7 @@
 // Caller memory may be host or device; host matrices are staged through the
 // device pool and only outputs are copied back.  Calls are synchronous.
 #include "kernels.h"
+#include "batched.h"
 #include <cctype>
 #include <vector>
 
@@ -192,8 +193,7 @@ void gemm_batch_impl(const char* name, int order, const int* ta_arr, const int* 
                      const blasint* n_arr, const blasint* k_arr, const T* alpha_arr, const T** a_arr,
                      const blasint* lda_arr, const T** b_arr, const blasint* ldb_arr, const T* beta_arr,
                      T** c_arr, const blasint* ldc_arr, blasint group_count, const blasint* group_size) {
-  struct P { int ta, tb; long m, n, k, lda, ldb, ldc; T alpha, beta; const T *a, *b; T* c; };
-  std::vector<P> probs;
+  std::vector<BDesc> probs;
   long idx = 0;
   for (blasint i = 0; i < group_count; idx += group_size[i], i++) {
     int ta = -1, tb = -1;
@@ -221,13 +221,20 @@ void gemm_batch_impl(const char* name, int order, const int* ta_arr, const int* 
     if (info >= 0) { report_xerbla(name, info); return; }
     if (M == 0 || N == 0) continue;
     for (blasint j = 0; j < group_size[i]; j++) {
-      P p{ta, tb, M, N, K, LDA, LDB, LDC, alpha_arr[i], beta_arr[i],
-          order == ColMajor ? a_arr[idx + j] : b_arr[idx + j],
-          order == ColMajor ? b_arr[idx + j] : a_arr[idx + j], c_arr[idx + j]};
-      probs.push_back(p);
+      BDesc d{};
+      d.a = (void*)(order == ColMajor ? a_arr[idx + j] : b_arr[idx + j]);
+      d.b = (void*)(order == ColMajor ? b_arr[idx + j] : a_arr[idx + j]);
+      d.c = (void*)c_arr[idx + j];
+      d.m = (int)M; d.n = (int)N; d.k = (int)K;
+      d.lda = (int)LDA; d.ldb = (int)LDB; d.ldc = (int)LDC;
+      d.idx = (int)(idx + j);
+      d.flags = ta | (tb << 1);
+      d.alpha = (double)alpha_arr[i];
+      d.beta = (double)beta_arr[i];
+      probs.push_back(d);
     }
   }
-  for (auto& p : probs) gemm_run<T>(p.ta, p.tb, p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb, p.beta, p.c, p.ldc);
+  gemm_batch_run<T>(probs);
 }
 
 }  // namespace

@@ -28,6 +28,7 @@
 //    interchange, which yields the composed permutation for the solves.
 #include "kernels.h"
 #include "lu_internal.h"
+#include "lu_device.h"
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -39,9 +40,6 @@ namespace oba {
 constexpr int LU_W = 32;        // leaf panel width
 constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
 
-template <typename T> struct MinNormal;
-template <> struct MinNormal<double> { static constexpr double v = DBL_MIN; };
-template <> struct MinNormal<float> { static constexpr float v = FLT_MIN; };
 
 // ---- transposes (optionally demoting, with an overflow flag) --------------
 template <typename Ti, typename To>
@@ -99,27 +97,6 @@ constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pair
 // rows per thread: FP64 rows take twice the registers
 template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
 
-// a panel row lives in registers as one vector value; dynamic element
-// indices go through static selects so the row never spills to scratch
-template <typename T, int W> struct RowVec { typedef T type __attribute__((ext_vector_type(W))); };
-
-template <typename V, int W>
-__device__ __attribute__((always_inline)) inline auto pick(const V& x, int j) {
-  auto v = x[0];
-#pragma unroll
-  for (int c = 1; c < W; c++) v = (c == j) ? x[c] : v;
-  return v;
-}
-
-// pivot key: |x|; a NaN only wins when it sits at the current position j
-// (IAMAX returns the first element when that one is NaN, and otherwise never
-// selects a NaN -- kernel/arm/iamax.c:51-74)
-template <typename T>
-__device__ inline double pivot_key(T xv, int pos, int j) {
-  return isnan(xv) ? (pos == j ? (double)INFINITY : -1.0) : fabs((double)xv);
-}
-
-__device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
 
 // candidate records of one column: [2 buffers][LU_PMAX] positions, row
 // numbers and rows (double-buffered by column parity: a workgroup can only
@@ -142,7 +119,6 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   const long mp = m - j0;
   __shared__ T srow[LU_PMAX * W];
   __shared__ int spos[LU_PMAX], sphys[LU_PMAX];
-  __shared__ T u[W];
   __shared__ double s_v[NWAVE];
   __shared__ int s_p[NWAVE], s_o[NWAVE];
   __shared__ int s_win, s_bad, s_p2p[W];

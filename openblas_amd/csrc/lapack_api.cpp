@@ -58,14 +58,14 @@ void perm_from_ipiv(const int* ipiv, long n, DevBuf& buf, int** perm_dev) {
 
 template <typename T> struct Names;
 template <> struct Names<double> {
-  static constexpr const char* getrf = "DGETRF"; static constexpr const char* getf2 = "DGETF2";
+  static constexpr const char* getrf = "DGETRF";
   static constexpr const char* getrs = "DGETRS"; static constexpr const char* gesv = "DGESV ";
-  static constexpr const char* potrf = "DPOTRF"; static constexpr const char* potf2 = "DPOTF2";
+  static constexpr const char* potrf = "DPOTRF";
 };
 template <> struct Names<float> {
-  static constexpr const char* getrf = "SGETRF"; static constexpr const char* getf2 = "SGETF2";
+  static constexpr const char* getrf = "SGETRF";
   static constexpr const char* getrs = "SGETRS"; static constexpr const char* gesv = "SGESV ";
-  static constexpr const char* potrf = "SPOTRF"; static constexpr const char* potf2 = "SPOTF2";
+  static constexpr const char* potrf = "SPOTRF";
 };
 
 template <typename T>

@@ -1,0 +1,36 @@
+// Device helpers shared by the LU kernels (single-problem panel and batched).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+
+namespace oba {
+
+template <typename T> struct MinNormal;
+template <> struct MinNormal<double> { static constexpr double v = DBL_MIN; };
+template <> struct MinNormal<float> { static constexpr float v = FLT_MIN; };
+
+// a panel row lives in registers as one vector value; dynamic element
+// indices go through static selects so the row never spills to scratch
+template <typename T, int W> struct RowVec { typedef T type __attribute__((ext_vector_type(W))); };
+
+template <typename V, int W>
+__device__ __attribute__((always_inline)) inline auto pick(const V& x, int j) {
+  auto v = x[0];
+#pragma unroll
+  for (int c = 1; c < W; c++) v = (c == j) ? x[c] : v;
+  return v;
+}
+
+// pivot key: |x|; a NaN only wins when it sits at the current position j
+// (IAMAX returns the first element when that one is NaN, and otherwise never
+// selects a NaN -- kernel/arm/iamax.c:51-74)
+template <typename T>
+__device__ inline double pivot_key(T xv, int pos, int j) {
+  return isnan(xv) ? (pos == j ? (double)INFINITY : -1.0) : fabs((double)xv);
+}
+
+// larger key wins; equal keys -> lower position (IAMAX: first maximum)
+__device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
+
+}  // namespace oba

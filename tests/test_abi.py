@@ -170,3 +170,30 @@ def test_python_layer_refuses_missing_library(tmp_path, monkeypatch):
     monkeypatch.setattr(oa, "_lib", None)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         oa.lib()
+
+
+def test_batched_argument_errors_without_device(L):
+    """Argument checks of the batched LAPACK entry points happen before any
+    device work: each problem gets the single routine's info numbering."""
+    import ctypes
+    i32 = lambda v: np.ascontiguousarray(np.array(v, np.int32))
+    pi = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    nul = (ctypes.c_void_p * 3)(0, 0, 0)
+    info = i32([9, 9, 9])
+    oa.last_xerbla()
+    # getrf: m < 0 -> -1, n < 0 -> -2, lda < max(1, m) -> -4
+    assert L.openblas_amd_dgetrf_batch(pi(i32([-1, 3, 5])), pi(i32([2, -1, 5])), nul, pi(i32([1, 3, 4])), nul,
+                                       pi(info), 3) == 0
+    assert list(info) == [-1, -2, -4] and oa.last_xerbla() == ("DGETRF", 1)
+    assert L.openblas_amd_dgetrf_batch(pi(info), pi(info), nul, pi(info), nul, pi(info), -1) == -7
+    # getrs: bad trans is argument 1 of the batched call
+    assert L.openblas_amd_dgetrs_batch(b"X", pi(info), pi(info), nul, pi(info), nul, nul, pi(info), pi(info), 1) == -1
+    info = i32([9, 9])
+    assert L.openblas_amd_sgetrs_batch(b"n", pi(i32([4, 4])), pi(i32([-1, 1])), nul, pi(i32([4, 4])), nul, nul,
+                                       pi(i32([4, 3])), pi(info), 2) == 0
+    assert list(info) == [-3, -8]
+    # potrf: lda < n -> -4
+    info = i32([9])
+    assert L.openblas_amd_dpotrf_batch(b"L", pi(i32([6])), nul, pi(i32([5])), pi(info), 1) == 0
+    assert list(info) == [-4]
+    assert L.openblas_amd_spotrf_batch(b"Q", pi(info), nul, pi(info), pi(info), 1) == -1
