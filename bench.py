@@ -30,6 +30,8 @@ import openblas_amd as oa  # noqa: E402
 
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 MFMA, published
 PEAK_FP32 = 157.3e12
+PEAK_INT8 = 5.0e15
+PEAK_HBM = 8.0e12
 
 
 def log(*a):
@@ -112,6 +114,227 @@ def cpu_baseline(A_host, b, budget_s=30.0):
     return res
 
 
+# ---- INT8-slice emulated DGEMM (M=N=K=8192, entries u*exp(phi*g)) -----------------
+def bench_ozaki(L, n=8192, phis=(0.5, 2.0, 8.0), reps=3, cpu=True):
+    res = []
+    for phi in phis:
+        rng = np.random.default_rng(int(phi * 100) + 11)
+        A = np.asfortranarray(rng.uniform(-0.5, 0.5, (n, n)) * np.exp(phi * rng.standard_normal((n, n))))
+        B = np.asfortranarray(rng.uniform(-0.5, 0.5, (n, n)) * np.exp(phi * rng.standard_normal((n, n))))
+        dA, dB = oa.DeviceBuffer.from_array(A), oa.DeviceBuffer.from_array(B)
+        dC, dC64 = oa.DeviceBuffer(8 * n * n), oa.DeviceBuffer(8 * n * n)
+        L.cblas_dgemm(102, 111, 111, n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC64.ptr, n)
+        t64 = []
+        for _ in range(reps):
+            L.openblas_amd_timer_start()
+            L.cblas_dgemm(102, 111, 111, n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC64.ptr, n)
+            t64.append(L.openblas_amd_timer_stop())
+        s = L.openblas_amd_dgemm_ozaki(102, 111, 111, n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr, n, 0)
+        tz = []
+        for _ in range(reps):
+            L.openblas_amd_timer_start()
+            L.openblas_amd_dgemm_ozaki(102, 111, 111, n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr, n, 0)
+            tz.append(L.openblas_amd_timer_stop())
+        C = np.empty((n, n), order="F"); C64 = np.empty((n, n), order="F")
+        dC.to_array(C); dC64.to_array(C64)
+        rows = rng.choice(n, 32, replace=False)
+        ref = A[rows] @ B
+        gauge = np.abs(A[rows]) @ np.abs(B)
+        err_oz = float(np.max(np.abs(C[rows] - ref) / gauge))
+        err_64 = float(np.max(np.abs(C64[rows] - ref) / gauge))
+        ms_z, ms_64 = float(np.median(tz)), float(np.median(t64))
+        fl = 2.0 * n ** 3
+        pairs = s * (s + 1) // 2
+        r = {"phi": phi, "slices": s, "ms": ms_z, "fp64_equiv_tflops": fl / ms_z / 1e9,
+             "frac_of_fp64_peak": fl / (ms_z * 1e-3) / PEAK_FP64,
+             "int8_tops": pairs * fl / ms_z / 1e9, "frac_of_int8_peak": pairs * fl / (ms_z * 1e-3) / PEAK_INT8,
+             "native_dgemm_ms": ms_64, "speedup_vs_native": ms_64 / ms_z,
+             "err_vs_ref_normwise": err_oz, "native_err_vs_ref_normwise": err_64}
+        if cpu:
+            try:
+                from oracle import refblas
+                refblas.set_num_threads(os.cpu_count() or 1)
+                Cs = np.zeros((1024, n), order="F")
+                As = np.asfortranarray(A[:1024])
+                t0 = time.perf_counter()
+                refblas.cblas_gemm(False, False, 1.0, As, B, 0.0, Cs)
+                r["cpu_dgemm_1024rows_s"] = time.perf_counter() - t0
+                r["cpu_dgemm_tflops"] = 2.0 * 1024 * n * n / r["cpu_dgemm_1024rows_s"] / 1e12
+            except Exception as e:
+                r["cpu"] = f"absent: {e}"
+        log(f"ozaki phi={phi}: s={s} {ms_z:.1f} ms ({r['fp64_equiv_tflops']:.1f} TF/s eq, native {ms_64:.1f} ms) "
+            f"err={err_oz:.2e} native_err={err_64:.2e}")
+        res.append(r)
+        for d in (dA, dB, dC, dC64):
+            d.free()
+        del A, B, C, C64
+    return res
+
+
+# ---- batched small problems (10,000 problems, n ~ U{8..256}) ----------------------
+def _ptrs(base, off, es):
+    return (ctypes.c_void_p * len(off))(*[base + int(o) * es for o in off])
+
+
+def _i32(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+
+
+def _pi(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def bench_batched(L, count=10000, seed=5, reps=3, cpu=True, loop_sample=300, cpu_sample=1000):
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(8, 257, count).astype(np.int64)
+    sq = ns * ns
+    off = np.concatenate([[0], np.cumsum(sq)[:-1]])
+    voff = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    total, vtotal = int(sq.sum()), int(ns.sum())
+    n32 = _i32(ns)
+    gemm_flops = float((2 * ns ** 3).sum())
+    lu_flops = float((2.0 / 3.0 * ns ** 3).sum())
+    rs_flops = float((2 * ns ** 2).sum())
+    po_flops = float((ns ** 3 / 3.0).sum())
+    out = {"count": count, "n_dist": "U{8..256}", "seed": seed}
+    for tag, dt in (("d", np.float64), ("s", np.float32)):
+        es = np.dtype(dt).itemsize
+        r = {}
+        A = rng.standard_normal(total).astype(dt)
+        Bm = rng.standard_normal(total).astype(dt)
+        dA, dB, dC, dW = (oa.DeviceBuffer.from_array(A), oa.DeviceBuffer.from_array(Bm), oa.DeviceBuffer(es * total),
+                          oa.DeviceBuffer(es * total))
+        pA, pB, pC, pW = (_ptrs(d.ptr, off, es) for d in (dA, dB, dC, dW))
+        ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
+        gemm_b = L.cblas_dgemm_batch if tag == "d" else L.cblas_sgemm_batch
+        one = (ct * count)(*([1.0] * count)); zero = (ct * count)(*([0.0] * count))
+        tn = _i32(np.full(count, 111)); tt = _i32(np.full(count, 112)); gs = _i32(np.ones(count))
+
+        def timed(fn, prep=None):
+            fn()   # first call: tuning + graph capture
+            ts, ws = [], []
+            for _ in range(reps):
+                if prep:
+                    prep()
+                L.openblas_amd_synchronize()
+                w0 = time.perf_counter()
+                L.openblas_amd_timer_start()
+                fn()
+                ts.append(L.openblas_amd_timer_stop())
+                ws.append((time.perf_counter() - w0) * 1e3)
+            return float(np.median(ts)), float(np.median(ws))
+
+        # (i) GEMM batch: group_count = count, group_size 1, C = A*B
+        g = lambda: gemm_b(102, _pi(tn), _pi(tn), _pi(n32), _pi(n32), _pi(n32), one, pA, _pi(n32), pB, _pi(n32), zero,
+                           pC, _pi(n32), count, _pi(gs))
+        ms, wall = timed(g)
+        byt = float((3 * sq).sum()) * es
+        r["gemm"] = {"ms": ms, "wall_ms": wall, "tflops": gemm_flops / ms / 1e9,
+                     "frac_of_peak": gemm_flops / (ms * 1e-3) / (PEAK_FP64 if tag == "d" else PEAK_FP32),
+                     "hbm_frac": byt / (ms * 1e-3) / PEAK_HBM}
+        # per-problem launch loop on a sample
+        f1 = L.cblas_dgemm if tag == "d" else L.cblas_sgemm
+        L.openblas_amd_synchronize()
+        w0 = time.perf_counter()
+        for i in range(loop_sample):
+            n = int(ns[i])
+            f1(102, 111, 111, n, n, n, 1.0, pA[i], n, pB[i], n, 0.0, pC[i], n)
+        L.openblas_amd_synchronize()
+        r["gemm"]["loop_ms_est"] = (time.perf_counter() - w0) * 1e3 * count / loop_sample
+
+        # (ii) getrf + getrs batch (A ~ N(0,1), b ~ N(0,1), 1 RHS)
+        dpiv = oa.DeviceBuffer(4 * vtotal)
+        ppiv = _ptrs(dpiv.ptr, voff, 4)
+        dbv, dx = oa.DeviceBuffer.from_array(rng.standard_normal(vtotal).astype(dt)), oa.DeviceBuffer(es * vtotal)
+        pX = _ptrs(dx.ptr, voff, es)
+        info = np.zeros(count, np.int32)
+        fr = L.openblas_amd_dgetrf_batch if tag == "d" else L.openblas_amd_sgetrf_batch
+        fs = L.openblas_amd_dgetrs_batch if tag == "d" else L.openblas_amd_sgetrs_batch
+        reset_a = lambda: L.openblas_amd_memcpy(dW.ptr, dA.ptr, es * total, 2)
+        ms, wall = timed(lambda: fr(_pi(n32), _pi(n32), pW, _pi(n32), ppiv, _pi(info), count), reset_a)
+        r["getrf"] = {"ms": ms, "wall_ms": wall, "tflops": lu_flops / ms / 1e9,
+                      "frac_of_peak": lu_flops / (ms * 1e-3) / (PEAK_FP64 if tag == "d" else PEAK_FP32),
+                      "hbm_frac": float((2 * sq).sum()) * es / (ms * 1e-3) / PEAK_HBM, "info_nonzero": int((info != 0).sum())}
+        reset_b = lambda: L.openblas_amd_memcpy(dx.ptr, dbv.ptr, es * vtotal, 2)
+        one_rhs = _i32(np.ones(count))
+        ms, wall = timed(lambda: fs(b"N", _pi(n32), _pi(one_rhs), pW, _pi(n32), ppiv, pX, _pi(n32), _pi(info), count),
+                         reset_b)
+        r["getrs"] = {"ms": ms, "wall_ms": wall, "gflops": rs_flops / ms / 1e6,
+                      "hbm_frac": float(sq.sum()) * es / (ms * 1e-3) / PEAK_HBM}
+        # residual spot check on 16 problems
+        Xh = np.empty(vtotal, dt); dx.to_array(Xh)
+        bh = np.empty(vtotal, dt); dbv.to_array(bh)
+        worst = 0.0
+        for i in rng.choice(count, 16, replace=False):
+            n, o, v = int(ns[i]), int(off[i]), int(voff[i])
+            Ai = A[o:o + n * n].reshape(n, n, order="F").astype(np.float64)
+            xi, bi = Xh[v:v + n].astype(np.float64), bh[v:v + n].astype(np.float64)
+            worst = max(worst, float(np.abs(bi - Ai @ xi).max() / (np.abs(Ai).sum(1).max() * np.abs(xi).max())))
+        r["getrs"]["worst_backward_error_16"] = worst
+        # per-problem loop (dgetrf_ on device pointers) on a sample
+        f1 = L.dgetrf_ if tag == "d" else L.sgetrf_
+        reset_a()
+        L.openblas_amd_synchronize()
+        ii = ctypes.c_int(0)
+        w0 = time.perf_counter()
+        for i in range(loop_sample):
+            n = int(ns[i])
+            f1(oa.ip(n), oa.ip(n), pW[i], oa.ip(n), ppiv[i], ctypes.byref(ii))
+        r["getrf"]["loop_ms_est"] = (time.perf_counter() - w0) * 1e3 * count / loop_sample
+
+        # (iii) potrf batch on A = B*B^T + n*I (built with the batched GEMM)
+        Ch = np.zeros(total, dt)
+        for i in range(count):
+            n, o = int(ns[i]), int(off[i])
+            Ch[o:o + n * n:n + 1] = n
+        dS = oa.DeviceBuffer.from_array(Ch)
+        del Ch
+        pS = _ptrs(dS.ptr, off, es)
+        gemm_b(102, _pi(tn), _pi(tt), _pi(n32), _pi(n32), _pi(n32), one, pB, _pi(n32), pB, _pi(n32), one, pS,
+               _pi(n32), count, _pi(gs))
+        fp = L.openblas_amd_dpotrf_batch if tag == "d" else L.openblas_amd_spotrf_batch
+        reset_s = lambda: L.openblas_amd_memcpy(dW.ptr, dS.ptr, es * total, 2)
+        ms, wall = timed(lambda: fp(b"L", _pi(n32), pW, _pi(n32), _pi(info), count), reset_s)
+        r["potrf"] = {"ms": ms, "wall_ms": wall, "tflops": po_flops / ms / 1e9,
+                      "frac_of_peak": po_flops / (ms * 1e-3) / (PEAK_FP64 if tag == "d" else PEAK_FP32),
+                      "info_nonzero": int((info != 0).sum())}
+
+        if cpu and tag == "d":
+            try:
+                from oracle import refblas
+                cores = os.cpu_count() or 1
+                refblas.set_num_threads(cores)
+                sel = range(cpu_sample)
+                As = [A[off[i]:off[i] + sq[i]].reshape(ns[i], ns[i], order="F").copy(order="F") for i in sel]
+                Bs = [Bm[off[i]:off[i] + sq[i]].reshape(ns[i], ns[i], order="F").copy(order="F") for i in sel]
+                Cs = [np.zeros((ns[i], ns[i]), order="F") for i in sel]
+                t0 = time.perf_counter()
+                refblas.dgemm_batch(As, Bs, Cs)
+                tg = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                for a in As:
+                    pv, _ = refblas.getrf(a)
+                    refblas.getrs(False, a, pv, np.ones((a.shape[0], 1), order="F"))
+                tl = time.perf_counter() - t0
+                Ss = [np.asfortranarray(b @ b.T + b.shape[0] * np.eye(b.shape[0])) for b in Bs]
+                t0 = time.perf_counter()
+                for a in Ss:
+                    refblas.potrf(True, a)
+                tp = time.perf_counter() - t0
+                scale = count / cpu_sample
+                r["cpu"] = {"cores": cores, "kind": "reference", "sample": f"first {cpu_sample} problems, scaled",
+                            "gemm_batch_ms_est": tg * 1e3 * scale, "getrf_getrs_loop_ms_est": tl * 1e3 * scale,
+                            "potrf_loop_ms_est": tp * 1e3 * scale}
+            except Exception as e:
+                r["cpu"] = f"absent: {e}"
+        log(f"batched {tag}: " + json.dumps({k: (v if not isinstance(v, dict) else {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}) for k, v in r.items()}))
+        out[tag] = r
+        for d in (dA, dB, dC, dW, dpiv, dbv, dx, dS):
+            d.free()
+        del A, Bm
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +344,8 @@ def main():
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ozaki", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
@@ -194,6 +419,9 @@ def main():
     log(f"dgemm 8192: {g_ms:.2f} ms {g_flops / 1e12:.1f} TF/s; dgemm 1024: {g1_flops / 1e12:.1f} TF/s; "
         f"sgemm 8192: {s_flops / 1e12:.1f} TF/s")
 
+    ozaki = None if args.no_ozaki else bench_ozaki(L, cpu=not args.no_cpu)
+    batched = None if args.no_batched else bench_batched(L, cpu=not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         log("timing the reference CPU path ...")
@@ -231,6 +459,8 @@ def main():
                      "achieved": g_flops / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": g_flops / PEAK_FP64, "traffic": None},
         "cpu_baseline": cpu,
+        "ozaki_dgemm_8192": ozaki,
+        "batched_10000": batched,
     }
     print(json.dumps(out), flush=True)
     for d in (dA, db, dX, dpiv):
