@@ -154,7 +154,10 @@ def dsgesv(A, B):
 
 
 def dgemm_batch(As, Bs, Cs, alpha=1.0, beta=0.0):
-    """cblas_dgemm_batch with one group per problem (NN, column-major)."""
+    """cblas_dgemm_batch with one group per problem (NN, column-major).
+    The installed 0.3.28 build crashes (SIGSEGV) when a problem has
+    m*n*k <= 100^3 (the small-matrix kernel route of
+    interface/gemm_batch.c:300-318): call it with larger problems only."""
     I = _it()
     cnt = len(As)
     it = np.int64 if _ilp64 else np.int32
