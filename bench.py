@@ -34,6 +34,18 @@ PEAK_INT8 = 5.0e15
 PEAK_HBM = 8.0e12
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated with a copy of known size)."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -349,6 +361,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ozaki", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--section", default="all", choices=["all", "solve", "dgemm", "ozaki", "batched"],
+                    help="run one workload only (for profiling); prints a partial JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
@@ -360,6 +374,26 @@ def main():
     ncu, clk = ctypes.c_int(0), ctypes.c_int(0)
     L.openblas_amd_device_info(arch, 64, ctypes.byref(ncu), ctypes.byref(clk))
     n, nrhs = args.n, args.nrhs
+    if args.section == "dgemm":
+        g_ms, g_flops = time_dgemm(L, 8192, np.float64, 5)
+        # a device-to-device copy of known size: the FETCH_SIZE / WRITE_SIZE
+        # calibration for the counter runs of this section
+        nb = 512 << 20
+        c1, c2 = oa.DeviceBuffer(nb), oa.DeviceBuffer(nb)
+        L.openblas_amd_memcpy(c2.ptr, c1.ptr, nb, 2)
+        L.openblas_amd_synchronize()
+        print(json.dumps({"section": "dgemm", "dgemm_8192_ms": g_ms, "tflops": g_flops / 1e12,
+                          "calibration_copy_bytes": nb}), flush=True)
+        return
+    if args.section == "ozaki":
+        print(json.dumps({"section": "ozaki", "ozaki": bench_ozaki(L, cpu=False, reps=2)}), flush=True)
+        return
+    if args.section == "batched":
+        print(json.dumps({"section": "batched", "batched": bench_batched(L, cpu=False, reps=2, loop_sample=50)}),
+              flush=True)
+        return
+    if args.section == "solve":
+        args.no_ozaki = args.no_batched = args.no_cpu = True
     log(f"device {arch.value.decode()} CUs={ncu.value} clock={clk.value // 1000} MHz; building A (n={n}, kappa={args.kappa:g})")
 
     dA, b = make_system(L, n, args.kappa, nrhs, seed=2024)
@@ -416,11 +450,13 @@ def main():
         d.free()
 
     # --- DGEMM roofline record (8192^3 NN, device-resident) and 1024^3 ---
-    g_ms, g_flops = time_dgemm(L, 8192, np.float64, 5)
-    g1_ms, g1_flops = time_dgemm(L, 1024, np.float64, 20)
-    s_ms, s_flops = time_dgemm(L, 8192, np.float32, 5)
-    log(f"dgemm 8192: {g_ms:.2f} ms {g_flops / 1e12:.1f} TF/s; dgemm 1024: {g1_flops / 1e12:.1f} TF/s; "
-        f"sgemm 8192: {s_flops / 1e12:.1f} TF/s")
+    g_ms = g_flops = g1_flops = s_flops = float("nan")
+    if args.section == "all":
+        g_ms, g_flops = time_dgemm(L, 8192, np.float64, 5)
+        g1_ms, g1_flops = time_dgemm(L, 1024, np.float64, 20)
+        s_ms, s_flops = time_dgemm(L, 8192, np.float32, 5)
+        log(f"dgemm 8192: {g_ms:.2f} ms {g_flops / 1e12:.1f} TF/s; dgemm 1024: {g1_flops / 1e12:.1f} TF/s; "
+            f"sgemm 8192: {s_flops / 1e12:.1f} TF/s")
 
     ozaki = None if args.no_ozaki else bench_ozaki(L, cpu=not args.no_cpu)
     batched = None if args.no_batched else bench_batched(L, cpu=not args.no_cpu)
@@ -460,7 +496,8 @@ def main():
         "sgemm_8192_tflops": s_flops / 1e12,
         "roofline": {"kernel": "gemm_kernel<double,128,128,16> (dgemm 8192^3 NN)", "bound": "mfma",
                      "achieved": g_flops / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": g_flops / PEAK_FP64, "traffic": None},
+                     "frac": g_flops / PEAK_FP64, "traffic": profiled_traffic("dgemm_8192_nn_fp64.md"),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)"},
         "cpu_baseline": cpu,
         "ozaki_dgemm_8192": ozaki,
         "batched_10000": batched,
