@@ -20,12 +20,30 @@
 
 namespace oba {
 
-// row sums of |A| (one thread per row: coalesced down the columns)
-__global__ void row_abs_sum_kernel(long n, const double* __restrict__ a, long lda, double* __restrict__ rs) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+// row sums of |A|: thread per row (coalesced down the columns), the columns
+// split into RS_CH chunks over blockIdx.y for parallelism; the partial sums
+// are added in a fixed order (deterministic)
+constexpr int RS_CH = 32;
+__global__ void row_abs_sum_part_kernel(long n, const double* __restrict__ a, long lda, double* __restrict__ part) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long c0 = n * blockIdx.y / RS_CH, c1 = n * (blockIdx.y + 1) / RS_CH;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  long j = c0;
+  for (; j + 4 <= c1; j += 4) {
+    s0 += fabs(a[i + j * lda]);
+    s1 += fabs(a[i + (j + 1) * lda]);
+    s2 += fabs(a[i + (j + 2) * lda]);
+    s3 += fabs(a[i + (j + 3) * lda]);
+  }
+  for (; j < c1; j++) s0 += fabs(a[i + j * lda]);
+  part[blockIdx.y * n + i] = (s0 + s1) + (s2 + s3);
+}
+__global__ void row_abs_sum_fin_kernel(long n, const double* __restrict__ part, double* __restrict__ rs) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (long j = 0; j < n; j++) s += fabs(a[i + j * lda]);
+  for (int c = 0; c < RS_CH; c++) s += part[c * n + i];
   rs[i] = s;
 }
 
@@ -115,8 +133,9 @@ void dsgesv_device(long n, long nrhs, double* a, long lda, int* ipiv_dev, const 
 
   // ANRM, CTE
   {
-    DevBuf rs(sizeof(double) * n);
-    row_abs_sum_kernel<<<nb256(n), 256, 0, s>>>(n, a, lda, rs.as<double>());
+    DevBuf rs(sizeof(double) * n), part(sizeof(double) * n * RS_CH);
+    row_abs_sum_part_kernel<<<dim3(nb256(n), RS_CH), 256, 0, s>>>(n, a, lda, part.as<double>());
+    row_abs_sum_fin_kernel<<<nb256(n), 256, 0, s>>>(n, part.as<double>(), rs.as<double>());
     max_nan_kernel<<<1, 256, 0, s>>>(n, rs.as<double>(), d_anrm);
     OBA_HIP(hipGetLastError());
   }

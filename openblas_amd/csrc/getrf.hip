@@ -83,20 +83,21 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // Factors A(j0:m, j0:j0+w) held as At(j0:j0+w, j0:m).  Workgroup g owns the
 // panel rows [g*ROWS, (g+1)*ROWS) in registers, thread t rows g*ROWS+q*NT+t.
 // Rows are never moved inside the kernel: each keeps its current position
-// (LAPACK order after the interchanges so far), so an interchange is an
-// O(1) update of two positions and the rows land at their final positions
-// when written back.  Per column, every workgroup publishes its best
-// candidate (position, row number, whole row); one grid barrier (release
-// fence, atomic arrival counter, acquire fence -- the cheapest exchange
-// measured on MI355X) makes all candidates visible, and every workgroup
-// reduces the same candidates to the same pivot, so no second exchange is
-// needed for the pivot row.
+// (LAPACK order after the interchanges so far) in a register, so an
+// interchange is an O(1) update of two positions and the rows land at their
+// final positions when written back.  Per column: a wavefront max-reduction,
+// one workgroup barrier, the workgroup's best row is published; with one
+// workgroup (P == 1) it goes through LDS, otherwise as a record (position,
+// row number, whole row) followed by one grid barrier (release fence,
+// atomic arrival counter, acquire fence) executed by the publishing thread
+// alone; after a second workgroup barrier every wavefront reduces the P
+// records itself and takes the pivot row with one load per lane and
+// v_readlane broadcasts.  Two workgroup barriers per column in all.
 constexpr int LU_NT = 512;            // threads per workgroup
 constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
 
-// rows per thread: FP64 rows take twice the registers
-template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
-
+// rows per thread (registers: RPT * W values of T per thread)
+template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 2 : 4; };
 
 // candidate records of one column: [2 buffers][LU_PMAX] positions, row
 // numbers and rows (double-buffered by column parity: a workgroup can only
@@ -108,24 +109,34 @@ struct PanelRec {
   T* row;
 };
 
+__device__ inline double lane_bcast(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 template <typename T, int W, int RPT, int NT>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
                                                       int* __restrict__ ipiv, int* __restrict__ info,
                                                       int* __restrict__ moves, PanelRec<T> rec, unsigned* counter,
                                                       unsigned base, unsigned* err) {
+  static_assert(W <= 64, "one pivot-row element per lane");
   constexpr int ROWS = NT * RPT;
   constexpr int NWAVE = NT / 64;
-  const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const long mp = m - j0;
-  __shared__ T srow[LU_PMAX * W];
-  __shared__ int spos[LU_PMAX], sphys[LU_PMAX];
   __shared__ double s_v[NWAVE];
-  __shared__ int s_p[NWAVE], s_o[NWAVE];
-  __shared__ int s_win, s_bad, s_p2p[W];
+  __shared__ int s_p[NWAVE];
+  __shared__ T s_row[W];            // P == 1: the pivot row
+  __shared__ T s_wrow[NWAVE][W];    // P > 1: each wavefront's copy of the pivot row
+  __shared__ int s_bad;
 
-  typename RowVec<T, W>::type x[RPT];
+  PRow<T, W> x[RPT];
   int pos[RPT];
-  bool live[RPT];
+  bool live[RPT];   // not yet a pivot row
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const long r = (long)g * ROWS + q * NT + t;
@@ -133,127 +144,151 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     pos[q] = (int)r;
     const T* src = At + j0 + (j0 + r) * ldt;
 #pragma unroll
-    for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
+    for (int c = 0; c < W; c++) x[q].set(c, (live[q] && c < w) ? src[c] : T(0));
   }
-  if (t < W) s_p2p[t] = t;
   if (t == 0) s_bad = 0;
   __syncthreads();
 
+#pragma unroll 1
   for (int j = 0; j < w; j++) {
     const int buf = j & 1;
-    // (a) this workgroup's best candidate: max key, lowest position on ties
+    // (a) this thread's, then this wavefront's best: max key, lowest position
     double bv = -2.0;
-    int bp = INT_MAX, bo = -1;
+    int bp = INT_MAX;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       if (live[q]) {
-        double v = pivot_key((T)pick<decltype(x[q]), W>(x[q], j), pos[q], j);
-        if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; bo = t * RPT + q; }
+        const double v = pivot_key(x[q].get(j), pos[q], j);
+        if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; }
       }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      double ov = __shfl_xor(bv, off);
-      int op = __shfl_xor(bp, off), oo = __shfl_xor(bo, off);
-      if (better(ov, op, bv, bp)) { bv = ov; bp = op; bo = oo; }
+      const double ov = __shfl_xor(bv, off);
+      const int op = __shfl_xor(bp, off);
+      if (better(ov, op, bv, bp)) { bv = ov; bp = op; }
     }
-    if ((t & 63) == 0) { s_v[t >> 6] = bv; s_p[t >> 6] = bp; s_o[t >> 6] = bo; }
+    if (lane == 0) { s_v[wave] = bv; s_p[wave] = bp; }
     __syncthreads();
-    double wv_ = s_v[0];
-    int wp = s_p[0], wo = s_o[0];
+    // (b) the workgroup's best (every thread reduces the same values)
+    double wv = s_v[0];
+    int wp = s_p[0];
 #pragma unroll
     for (int k = 1; k < NWAVE; k++)
-      if (better(s_v[k], s_p[k], wv_, wp)) { wv_ = s_v[k]; wp = s_p[k]; wo = s_o[k]; }
-    // (b) publish this workgroup's candidate (plain stores)
-    const int slot = buf * LU_PMAX + g;
-    if (wo >= 0 && t == wo / RPT) {
-      const int qq = wo % RPT;
+      if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; }
+    int gp;      // pivot position
+    T uv;        // lane c < W holds element c of the pivot row
+    const T* urow;   // the pivot row in LDS
+    if (P == 1) {
 #pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        if (q == qq) {
-          rec.pos[slot] = pos[q];
-          rec.phys[slot] = (int)((long)g * ROWS + q * NT + t);
+      for (int q = 0; q < RPT; q++)
+        if (live[q] && pos[q] == wp) {
 #pragma unroll
-          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q][c];
+          for (int c = 0; c < W; c++) s_row[c] = x[q].get(c);
         }
+      __syncthreads();
+      gp = wp;
+      uv = lane < W ? s_row[lane] : T(0);
+      urow = s_row;
+    } else {
+      // (c) publish the candidate record; the publishing thread alone runs
+      // the grid barrier (release, arrive, bounded spin, acquire)
+      const int slot = buf * LU_PMAX + g;
+      bool pub = false;
+#pragma unroll
+      for (int q = 0; q < RPT; q++)
+        if (live[q] && pos[q] == wp) {
+          rec.pos[slot] = wp;
+          rec.phys[slot] = g * ROWS + q * NT + t;
+#pragma unroll
+          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q].get(c);
+          pub = true;
+        }
+      if (wp == INT_MAX && t == 0) {   // no live row left in this workgroup
+        rec.pos[slot] = INT_MAX;
+        rec.phys[slot] = 0;
+        pub = true;
       }
-    } else if (wo < 0 && t == 0) {
-      rec.pos[slot] = INT_MAX;
-      rec.phys[slot] = 0;
-    }
-    __syncthreads();
-    // (c) grid barrier: release, arrive, bounded spin, acquire
-    if (t == 0) {
-      __threadfence();
-      atomicAdd(counter, 1u);
-      const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
-      long spins = 0;
-      while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > (1L << 22)) { s_bad = 1; atomicExch(err, 1u); break; }
+      if (pub) {
+        __threadfence();
+        atomicAdd(counter, 1u);
+        const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
+        long spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          if (++spins > (1L << 22)) { s_bad = 1; atomicExch(err, 1u); break; }
+        }
+        __threadfence();
       }
-      __threadfence();
-    }
-    __syncthreads();
-    if (s_bad) break;
-    // (d) every workgroup reads all candidates and reduces them identically
-    for (int idx = t; idx < P * W; idx += NT) srow[idx] = rec.row[(size_t)buf * LU_PMAX * W + idx];
-    if (t < P) { spos[t] = rec.pos[buf * LU_PMAX + t]; sphys[t] = rec.phys[buf * LU_PMAX + t]; }
-    __syncthreads();
-    if (t < 64) {
+      __syncthreads();
+      if (s_bad) break;
+      // (d) every wavefront reduces the P records on its own
       double v = -2.0;
       int p = INT_MAX, wi = -1;
-      for (int c = t; c < P; c += 64) {
-        const int cp = spos[c];
+      for (int c = lane; c < P; c += 64) {
+        const int cp = rec.pos[buf * LU_PMAX + c];
         if (cp == INT_MAX) continue;
-        const double vv = pivot_key(srow[c * W + j], cp, j);
+        const double vv = pivot_key(rec.row[(size_t)(buf * LU_PMAX + c) * W + j], cp, j);
         if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off);
-        int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
+        const double ov = __shfl_xor(v, off);
+        const int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
         if (better(ov, op, v, p)) { v = ov; p = op; wi = ow; }
       }
-      if (t == 0) s_win = wi;
+      if (wi < 0 || wi >= P || p < j || p >= mp) {   // cannot happen: mp > j rows are live
+        if (t == 0) atomicExch(err, 2u);
+        break;
+      }
+      gp = p;
+      uv = lane < W ? rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane] : T(0);
+      if (lane < W) s_wrow[wave][lane] = uv;   // read back by this wavefront only
+      urow = s_wrow[wave];
     }
-    __syncthreads();
-    const int wi = s_win;
-    if (wi < 0 || wi >= P) { s_bad = 1; atomicExch(err, 2u); break; }   // cannot happen: mp > j rows are live
-    const int q_win = spos[wi];
-    const int phys_win = sphys[wi];
-    if (q_win < j || q_win >= mp || phys_win < 0 || phys_win >= mp) { s_bad = 1; atomicExch(err, 2u); break; }
-    const T* uw = srow + wi * W;
-    const T piv = uw[j];
+    const T piv = lane_bcast(uv, j);
     const bool nz = piv != T(0);
     const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
-    const int a = s_p2p[j];
     if (g == 0 && t == 0) {
-      ipiv[j0 + j] = (int)(j0 + q_win + 1);
+      ipiv[j0 + j] = (int)(j0 + gp + 1);
       if (!nz && *info == 0) *info = (int)(j0 + j + 1);
     }
     const T rinv = doscale ? T(1) / piv : T(1);
+    // interchange (positions only), multipliers, then the rank-1 update of the
+    // live columns j+1..w-1: one uniform runtime loop, each register row
+    // element addressed by GPR index mode (no per-element masks or branches)
+    T lq[RPT];
+    bool upd[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      const int r = g * ROWS + q * NT + t;
-      if (r == phys_win) { live[q] = false; pos[q] = j; }
-      else if (r == a) pos[q] = q_win;
-      if (live[q]) {
-        T l = pick<decltype(x[q]), W>(x[q], j);
-        if (doscale) l *= rinv;
+      upd[q] = false;
+      lq[q] = T(0);
+      if (!live[q]) continue;
+      if (pos[q] == gp) { pos[q] = j; live[q] = false; continue; }   // the pivot row: now a row of U
+      if (pos[q] == j) pos[q] = gp;
+      T l = x[q].get(j);
+      if (doscale) l *= rinv;
+      x[q].set(j, l);
+      lq[q] = l;
+      upd[q] = true;
+    }
+    // FP32: the pivot row in registers (vector LDS reads, indexed like x);
+    // FP64: read per column from LDS (the register file is full)
+    PRow<T, W> u;
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
-        for (int c = 0; c < W; c++) {
-          if (c == j) x[q][c] = l;
-          else if (c > j && c < w) x[q][c] -= l * uw[c];
-        }
+      for (int c = 0; c < W; c++) u.set(c, urow[c]);
+    }
+#pragma unroll 1
+    for (int c = j + 1; c < w; c++) {
+      const T uc = sizeof(T) == 4 ? u.get(c) : urow[c];
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const T xv = x[q].get(c);
+        x[q].set(c, upd[q] ? xv - lq[q] * uc : xv);
       }
     }
-    __syncthreads();
-    if (t == 0) {
-      s_p2p[j] = phys_win;
-      if (q_win < W) s_p2p[q_win] = a;
-    }
   }
-  if (s_bad) return;
+  if (s_bad || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   // every row to its final position; list the rows that moved
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
@@ -262,7 +297,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       T* dst = At + j0 + (j0 + pos[q]) * ldt;
 #pragma unroll
       for (int c = 0; c < W; c++)
-        if (c < w) dst[c] = x[q][c];
+        if (c < w) dst[c] = x[q].get(c);
       if (pos[q] != (int)r) {
         const int k = atomicAdd(moves, 1);
         if (k < 2 * W) {   // at most 2w rows change position
@@ -338,7 +373,7 @@ static void lu_panel(LuCtx<T>& c, long j0, int w) {
   lu_panel_kernel<T, LU_W, RPT, LU_NT><<<(unsigned)P, LU_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
                                                                         c.rec, c.counter, c.base, c.err);
   OBA_HIP(hipGetLastError());
-  c.base += (unsigned)P * (unsigned)w;
+  if (P > 1) c.base += (unsigned)P * (unsigned)w;   // one arrival per workgroup and column
   c.panel_no++;
   if (c.nrows_t - w > 0) {
     lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 63) / 64), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);

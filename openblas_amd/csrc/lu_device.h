@@ -22,6 +22,27 @@ __device__ __attribute__((always_inline)) inline auto pick(const V& x, int j) {
   return v;
 }
 
+// A panel row in registers with element access by a uniform runtime index,
+// which the compiler maps to GPR index mode (s_set_gpr_idx_on + v_mov)
+// instead of selects or scratch.  FP64 rows are kept as their low and high
+// 32-bit halves: index mode handles vectors of 32 dwords, not of 32 qwords.
+template <typename T, int W> struct PRow;
+template <int W> struct PRow<float, W> {
+  typedef float V __attribute__((ext_vector_type(W)));
+  V v;
+  __device__ __attribute__((always_inline)) float get(int c) const { return v[c]; }
+  __device__ __attribute__((always_inline)) void set(int c, float x) { v[c] = x; }
+};
+template <int W> struct PRow<double, W> {
+  typedef int V __attribute__((ext_vector_type(W)));
+  V lo, hi;
+  __device__ __attribute__((always_inline)) double get(int c) const { return __hiloint2double(hi[c], lo[c]); }
+  __device__ __attribute__((always_inline)) void set(int c, double x) {
+    lo[c] = __double2loint(x);
+    hi[c] = __double2hiint(x);
+  }
+};
+
 // pivot key: |x|; a NaN only wins when it sits at the current position j
 // (IAMAX returns the first element when that one is NaN, and otherwise never
 // selects a NaN -- kernel/arm/iamax.c:51-74)
