@@ -32,6 +32,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -91,13 +92,16 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // row number, whole row) followed by one grid barrier (release fence,
 // atomic arrival counter, acquire fence) executed by the publishing thread
 // alone; after a second workgroup barrier every wavefront reduces the P
-// records itself and takes the pivot row with one load per lane and
-// v_readlane broadcasts.  Two workgroup barriers per column in all.
+// records itself and copies the pivot row to its own LDS slot.  Two
+// workgroup barriers per column in all.  The column loop runs at run time,
+// but every register index is static: the update step of column j is
+// selected by a uniform switch (a per-element runtime index costs GPR index
+// mode, measured at ~10 us per column).
 constexpr int LU_NT = 512;            // threads per workgroup
 constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
 
 // rows per thread (registers: RPT * W values of T per thread)
-template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 2 : 4; };
+template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
 
 // candidate records of one column: [2 buffers][LU_PMAX] positions, row
 // numbers and rows (double-buffered by column parity: a workgroup can only
@@ -109,13 +113,30 @@ struct PanelRec {
   T* row;
 };
 
-__device__ inline double lane_bcast(double v, int l) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
+template <int J, typename T, int W, int RPT>
+__device__ __attribute__((always_inline)) inline void lu_col_step(T (&x)[RPT][W], const T (&lq)[RPT],
+                                                                   const bool (&upd)[RPT], const T* u,
+                                                                   T (&cand)[RPT]) {
+#pragma unroll
+  for (int q = 0; q < RPT; q++) x[q][J] = upd[q] ? lq[q] : x[q][J];
+#pragma unroll
+  for (int c = J + 1; c < W; c++) {
+    const T uc = u[c];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) x[q][c] = upd[q] ? x[q][c] - lq[q] * uc : x[q][c];
+  }
+  if constexpr (J + 1 < W) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) cand[q] = x[q][J + 1];
+  }
 }
-__device__ inline float lane_bcast(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+
+template <int J, int W, typename F>
+__device__ __attribute__((always_inline)) inline void lu_col_dispatch(int j, F&& f) {
+  if constexpr (J < W) {
+    if (j == J) f(std::integral_constant<int, J>{});
+    else lu_col_dispatch<J + 1, W>(j, f);
+  }
 }
 
 template <typename T, int W, int RPT, int NT>
@@ -134,7 +155,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   __shared__ T s_wrow[NWAVE][W];    // P > 1: each wavefront's copy of the pivot row
   __shared__ int s_bad;
 
-  PRow<T, W> x[RPT];
+  T x[RPT][W];
+  T cand[RPT];      // element j of each row: the pivot candidates of column j
   int pos[RPT];
   bool live[RPT];   // not yet a pivot row
 #pragma unroll
@@ -144,7 +166,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     pos[q] = (int)r;
     const T* src = At + j0 + (j0 + r) * ldt;
 #pragma unroll
-    for (int c = 0; c < W; c++) x[q].set(c, (live[q] && c < w) ? src[c] : T(0));
+    for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
+    cand[q] = x[q][0];
   }
   if (t == 0) s_bad = 0;
   __syncthreads();
@@ -158,7 +181,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       if (live[q]) {
-        const double v = pivot_key(x[q].get(j), pos[q], j);
+        const double v = pivot_key(cand[q], pos[q], j);
         if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; }
       }
     }
@@ -176,19 +199,17 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll
     for (int k = 1; k < NWAVE; k++)
       if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; }
-    int gp;      // pivot position
-    T uv;        // lane c < W holds element c of the pivot row
+    int gp;          // pivot position
     const T* urow;   // the pivot row in LDS
     if (P == 1) {
 #pragma unroll
       for (int q = 0; q < RPT; q++)
         if (live[q] && pos[q] == wp) {
 #pragma unroll
-          for (int c = 0; c < W; c++) s_row[c] = x[q].get(c);
+          for (int c = 0; c < W; c++) s_row[c] = x[q][c];
         }
       __syncthreads();
       gp = wp;
-      uv = lane < W ? s_row[lane] : T(0);
       urow = s_row;
     } else {
       // (c) publish the candidate record; the publishing thread alone runs
@@ -201,7 +222,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
           rec.pos[slot] = wp;
           rec.phys[slot] = g * ROWS + q * NT + t;
 #pragma unroll
-          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q].get(c);
+          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q][c];
           pub = true;
         }
       if (wp == INT_MAX && t == 0) {   // no live row left in this workgroup
@@ -241,11 +262,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
         break;
       }
       gp = p;
-      uv = lane < W ? rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane] : T(0);
-      if (lane < W) s_wrow[wave][lane] = uv;   // read back by this wavefront only
-      urow = s_wrow[wave];
+      if (lane < W) s_wrow[wave][lane] = rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane];
+      urow = s_wrow[wave];   // written and read by this wavefront only
     }
-    const T piv = lane_bcast(uv, j);
+    const T piv = urow[j];
     const bool nz = piv != T(0);
     const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
     if (g == 0 && t == 0) {
@@ -253,9 +273,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       if (!nz && *info == 0) *info = (int)(j0 + j + 1);
     }
     const T rinv = doscale ? T(1) / piv : T(1);
-    // interchange (positions only), multipliers, then the rank-1 update of the
-    // live columns j+1..w-1: one uniform runtime loop, each register row
-    // element addressed by GPR index mode (no per-element masks or branches)
+    // interchange (positions only) and multipliers; then the static step
     T lq[RPT];
     bool upd[RPT];
 #pragma unroll
@@ -265,28 +283,12 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       if (!live[q]) continue;
       if (pos[q] == gp) { pos[q] = j; live[q] = false; continue; }   // the pivot row: now a row of U
       if (pos[q] == j) pos[q] = gp;
-      T l = x[q].get(j);
-      if (doscale) l *= rinv;
-      x[q].set(j, l);
-      lq[q] = l;
+      lq[q] = doscale ? cand[q] * rinv : cand[q];
       upd[q] = true;
     }
-    // FP32: the pivot row in registers (vector LDS reads, indexed like x);
-    // FP64: read per column from LDS (the register file is full)
-    PRow<T, W> u;
-    if constexpr (sizeof(T) == 4) {
-#pragma unroll
-      for (int c = 0; c < W; c++) u.set(c, urow[c]);
-    }
-#pragma unroll 1
-    for (int c = j + 1; c < w; c++) {
-      const T uc = sizeof(T) == 4 ? u.get(c) : urow[c];
-#pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        const T xv = x[q].get(c);
-        x[q].set(c, upd[q] ? xv - lq[q] * uc : xv);
-      }
-    }
+    lu_col_dispatch<0, W>(j, [&](auto jc) __attribute__((always_inline)) {
+      lu_col_step<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow, cand);
+    });
   }
   if (s_bad || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   // every row to its final position; list the rows that moved
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       T* dst = At + j0 + (j0 + pos[q]) * ldt;
 #pragma unroll
       for (int c = 0; c < W; c++)
-        if (c < w) dst[c] = x[q].get(c);
+        if (c < w) dst[c] = x[q][c];
       if (pos[q] != (int)r) {
         const int k = atomicAdd(moves, 1);
         if (k < 2 * W) {   // at most 2w rows change position
