@@ -40,6 +40,7 @@ namespace oba {
 
 constexpr int LU_W = 32;        // leaf panel width
 constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
+constexpr int LU_TRSV_MAX_RHS = 4;   // solves with this many columns or fewer use trsv_lookback
 
 
 // ---- transposes (optionally demoting, with an overflow flag) --------------
@@ -89,9 +90,9 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // final positions when written back.  Per column: a wavefront max-reduction,
 // one workgroup barrier, the workgroup's best row is published; with one
 // workgroup (P == 1) it goes through LDS, otherwise as a record (position,
-// row number, whole row) followed by one grid barrier (release fence,
-// atomic arrival counter, acquire fence) executed by the publishing thread
-// alone; after a second workgroup barrier every wavefront reduces the P
+// row number, whole row) written through to memory, followed by one grid
+// barrier (completion wait, atomic arrival counter, spin) executed by the
+// publishing thread alone; after a second workgroup barrier every wavefront reduces the P
 // records itself and copies the pivot row to its own LDS slot.  Two
 // workgroup barriers per column in all.  The column loop runs at run time,
 // but every register index is static: the update step of column j is
@@ -113,6 +114,22 @@ struct PanelRec {
   T* row;
 };
 
+// Cross-workgroup records: write-through stores (to memory, past the
+// non-coherent L2) and loads that bypass the caches, so an exchange needs
+// no L2 writeback or invalidation (both measured expensive on this part).
+template <typename V>
+__device__ inline void wt_store(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+template <typename V>
+__device__ inline V bypass_load(const V* p) {
+  return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// One column step of the panel with the column index J known at compile
+// time: row q's multiplier goes to element J, elements J+1..W-1 take the
+// rank-1 update with the pivot row u (zero beyond w, so padding columns stay
+// finite), and element J+1 becomes the row's candidate for the next column.
+// Every register index is static: the column loop dispatches here through a
+// uniform switch instead of indexing registers at run time.
 template <int J, typename T, int W, int RPT>
 __device__ __attribute__((always_inline)) inline void lu_col_step(T (&x)[RPT][W], const T (&lq)[RPT],
                                                                    const bool (&upd)[RPT], const T* u,
@@ -219,26 +236,27 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll
       for (int q = 0; q < RPT; q++)
         if (live[q] && pos[q] == wp) {
-          rec.pos[slot] = wp;
-          rec.phys[slot] = g * ROWS + q * NT + t;
+          wt_store(&rec.pos[slot], wp);
+          wt_store(&rec.phys[slot], g * ROWS + q * NT + t);
 #pragma unroll
-          for (int c = 0; c < W; c++) rec.row[(size_t)slot * W + c] = x[q][c];
+          for (int c = 0; c < W; c++) wt_store(&rec.row[(size_t)slot * W + c], x[q][c]);
           pub = true;
         }
       if (wp == INT_MAX && t == 0) {   // no live row left in this workgroup
-        rec.pos[slot] = INT_MAX;
-        rec.phys[slot] = 0;
+        wt_store(&rec.pos[slot], INT_MAX);
+        wt_store(&rec.phys[slot], 0);
         pub = true;
       }
       if (pub) {
-        __threadfence();
+        // the record went out with write-through stores: wait for them to
+        // complete, then arrive (no L2 writeback needed)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         atomicAdd(counter, 1u);
         const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
         long spins = 0;
         while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
           if (++spins > (1L << 22)) { s_bad = 1; atomicExch(err, 1u); break; }
         }
-        __threadfence();
       }
       __syncthreads();
       if (s_bad) break;
@@ -246,9 +264,9 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       double v = -2.0;
       int p = INT_MAX, wi = -1;
       for (int c = lane; c < P; c += 64) {
-        const int cp = rec.pos[buf * LU_PMAX + c];
+        const int cp = bypass_load(&rec.pos[buf * LU_PMAX + c]);
         if (cp == INT_MAX) continue;
-        const double vv = pivot_key(rec.row[(size_t)(buf * LU_PMAX + c) * W + j], cp, j);
+        const double vv = pivot_key(bypass_load(&rec.row[(size_t)(buf * LU_PMAX + c) * W + j]), cp, j);
         if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
       }
 #pragma unroll
@@ -262,7 +280,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
         break;
       }
       gp = p;
-      if (lane < W) s_wrow[wave][lane] = rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane];
+      if (lane < W) s_wrow[wave][lane] = bypass_load(&rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane]);
       urow = s_wrow[wave];   // written and read by this wavefront only
     }
     const T piv = urow[j];
@@ -508,13 +526,28 @@ template <typename T>
 void getrs_perm(bool trans, long n, long nrhs, const T* a, long lda, const int* perm_dev, T* b, long ldb,
                 hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return;
+  const bool few = nrhs <= LU_TRSV_MAX_RHS;   // one launch per triangle and column
   if (!trans) {
     permute_rows<T>(false, n, nrhs, perm_dev, b, ldb, s);
-    trsm<T>(false, true, false, true, n, nrhs, T(1), a, lda, b, ldb, s);
-    trsm<T>(false, false, false, false, n, nrhs, T(1), a, lda, b, ldb, s);
+    if (few) {
+      for (long r = 0; r < nrhs; r++) {
+        trsv_lookback<T>(true, true, n, a, 1, lda, b + r * ldb, s);
+        trsv_lookback<T>(false, false, n, a, 1, lda, b + r * ldb, s);
+      }
+    } else {
+      trsm<T>(false, true, false, true, n, nrhs, T(1), a, lda, b, ldb, s);
+      trsm<T>(false, false, false, false, n, nrhs, T(1), a, lda, b, ldb, s);
+    }
   } else {
-    trsm<T>(false, false, true, false, n, nrhs, T(1), a, lda, b, ldb, s);
-    trsm<T>(false, true, true, true, n, nrhs, T(1), a, lda, b, ldb, s);
+    if (few) {
+      for (long r = 0; r < nrhs; r++) {   // U^T y = b (lower), then L^T x = y (upper, unit)
+        trsv_lookback<T>(true, false, n, a, lda, 1, b + r * ldb, s);
+        trsv_lookback<T>(false, true, n, a, lda, 1, b + r * ldb, s);
+      }
+    } else {
+      trsm<T>(false, false, true, false, n, nrhs, T(1), a, lda, b, ldb, s);
+      trsm<T>(false, true, true, true, n, nrhs, T(1), a, lda, b, ldb, s);
+    }
     permute_rows<T>(true, n, nrhs, perm_dev, b, ldb, s);
   }
 }
@@ -525,6 +558,13 @@ void getrs_t(long n, long nrhs, const T* At, long ldt, const int* perm_dev, T* b
   if (n <= 0 || nrhs <= 0) return;
   permute_rows<T>(false, n, nrhs, perm_dev, b, ldb, s);
   // L y = b: L = (unit upper part of At)^T ; U x = y: U = (lower part of At)^T
+  if (nrhs <= LU_TRSV_MAX_RHS) {
+    for (long r = 0; r < nrhs; r++) {   // L(i,k) = At[k + i*ldt], U(i,k) = At[k + i*ldt]
+      trsv_lookback<T>(true, true, n, At, ldt, 1, b + r * ldb, s);
+      trsv_lookback<T>(false, false, n, At, ldt, 1, b + r * ldb, s);
+    }
+    return;
+  }
   trsm<T>(false, false, true, true, n, nrhs, T(1), At, ldt, b, ldb, s);
   trsm<T>(false, true, true, false, n, nrhs, T(1), At, ldt, b, ldb, s);
 }

@@ -32,6 +32,11 @@ template <typename T>
 void getrs_perm(bool trans, long n, long nrhs, const T* a, long lda, const int* perm_dev, T* b, long ldb,
                 hipStream_t s);
 
+// One right-hand side: M x = b in a single launch, M(i,k) = a[i*rs + k*cs]
+// (rs == 1 or cs == 1), lower -> forward, upper -> backward substitution.
+template <typename T>
+void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, hipStream_t s);
+
 template <typename T>
 void getrs_t(long n, long nrhs, const T* At, long ldt, const int* perm_dev, T* b, long ldb, hipStream_t s);
 

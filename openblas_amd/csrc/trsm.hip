@@ -132,13 +132,151 @@ __global__ __launch_bounds__(256) void trsm_leaf_rows_kernel(int nb, long nvec, 
     if (c < nb) B[v + c * ldb] = x[c];
 }
 
+// Right-side leaves through explicit inverses of the 64x64 diagonal blocks:
+// a fully unrolled per-row substitution is ~4000 straight-line instructions
+// run once per wavefront and measured at 40-65 us per leaf whatever its
+// size (instruction fetch); the two loop-based kernels below replace it.
+//
+// (1) One workgroup per diagonal block b of M = op(A) (upper or lower,
+//     unit or not) computes M_bb^{-1} by doubling: the four 16x16 diagonal
+//     blocks are inverted by substitution (one thread per column, the
+//     diagonal applied as a multiplication by its reciprocal as
+//     trsm_kernel_LT.c:95), then pairs are joined with two small products
+//     each (upper: Y12 = -Y11 M12 Y22; lower: Y21 = -Y22 M21 Y11), 16 -> 32
+//     -> 64.  Stored row-major at inv[b*64*64 + i*64 + j].
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_inv_diag_kernel(long na, const T* __restrict__ A, long lda, bool mT,
+                                                            bool mupper, bool unit, T* __restrict__ inv,
+                                                            const int* abort_flag) {
+  constexpr int LD = TRSM_NB + 1;
+  __shared__ T Ms[TRSM_NB * LD];   // M(i,k) at Ms[i*LD + k]
+  __shared__ T Ys[TRSM_NB * LD];   // M^{-1}(i,j) at Ys[i*LD + j]
+  __shared__ T Ts[TRSM_NB * LD];   // products of the joins
+  if (abort_flag && *abort_flag) return;
+  const long o = (long)blockIdx.x * TRSM_NB;
+  const int nb = (int)min((long)TRSM_NB, na - o);
+  const int t = threadIdx.x;
+  for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
+    const int f = idx & 63, g = idx >> 6;   // A(o+f, o+g), f contiguous
+    T v = T(0);
+    if (f < nb && g < nb) v = A[(o + f) + (o + g) * lda];
+    else if (f == g) v = T(1);   // pad with the identity
+    if (mT) Ms[g * LD + f] = v;  // M(g,f) = A(f,g)
+    else Ms[f * LD + g] = v;
+    Ys[f * LD + g] = T(0);
+  }
+  __syncthreads();
+  // 16x16 diagonal blocks: thread (blk, j) computes column j of block blk
+  if (t < TRSM_NB) {
+    const int blk = t >> 4, j = t & 15, o16 = blk * 16, cj = o16 + j;
+    auto dinv = [&](int i) { return unit ? T(1) : T(1) / Ms[i * LD + i]; };
+    Ys[cj * LD + cj] = dinv(cj);
+    if (mupper) {
+      for (int i = j - 1; i >= 0; i--) {
+        const int ri = o16 + i;
+        T acc = T(0);
+        for (int k = i + 1; k <= j; k++) acc += Ms[ri * LD + o16 + k] * Ys[(o16 + k) * LD + cj];
+        Ys[ri * LD + cj] = -acc * dinv(ri);
+      }
+    } else {
+      for (int i = j + 1; i < 16; i++) {
+        const int ri = o16 + i;
+        T acc = T(0);
+        for (int k = j; k < i; k++) acc += Ms[ri * LD + o16 + k] * Ys[(o16 + k) * LD + cj];
+        Ys[ri * LD + cj] = -acc * dinv(ri);
+      }
+    }
+  }
+  __syncthreads();
+  for (int h = 16; h < TRSM_NB; h *= 2) {
+    const int npair = TRSM_NB / (2 * h);
+    // T = M_off * Y_near   (upper: M12 Y22; lower: M21 Y11)
+    for (int idx = t; idx < npair * h * h; idx += 256) {
+      const int p = idx / (h * h), r = (idx / h) % h, c = idx % h, po = p * 2 * h;
+      const int mr = mupper ? po + r : po + h + r, mc0 = mupper ? po + h : po;   // M rows / first column
+      const int yc = mupper ? po + h + c : po + c;                                  // Y column
+      T acc = T(0);
+      for (int k = 0; k < h; k++) acc += Ms[mr * LD + mc0 + k] * Ys[(mc0 + k) * LD + yc];
+      Ts[(po + r) * LD + c] = acc;
+    }
+    __syncthreads();
+    // Y_off = -Y_far * T   (upper: Y12 = -Y11 T; lower: Y21 = -Y22 T)
+    for (int idx = t; idx < npair * h * h; idx += 256) {
+      const int p = idx / (h * h), r = (idx / h) % h, c = idx % h, po = p * 2 * h;
+      const int yr = mupper ? po + r : po + h + r, yk0 = mupper ? po : po + h;
+      T acc = T(0);
+      for (int k = 0; k < h; k++) acc += Ys[yr * LD + yk0 + k] * Ts[(po + k) * LD + c];
+      const int oc = mupper ? po + h + c : po + c;
+      Ys[yr * LD + oc] = -acc;
+    }
+    __syncthreads();
+  }
+  T* out = inv + (long)blockIdx.x * TRSM_NB * TRSM_NB;
+  for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
+    const int i = idx >> 6, j = idx & 63;
+    out[idx] = (i < nb && j < nb) ? Ys[i * LD + j] : T(0);
+  }
+}
+
+// (2) X <- X * M_bb^{-1} for the leaf's nb columns of B, in place: a
+//     workgroup stages 64 rows x nb columns of B and the inverse in LDS;
+//     wavefront w computes columns 16w..16w+15 of all 64 rows (the inverse
+//     row is a broadcast read, the B column a contiguous one).
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_right_inv_kernel(int nb, long nvec, const T* __restrict__ inv,
+                                                             T* __restrict__ B, long ldb, const int* abort_flag) {
+  constexpr int LDX = TRSM_NB + 1;
+  __shared__ T Xs[TRSM_NB * LDX];                              // X(r, k) at Xs[k*LDX + r]
+  __shared__ __attribute__((aligned(16))) T Ys[TRSM_NB * TRSM_NB];   // inverse, row-major
+  if (abort_flag && *abort_flag) return;
+  const long v0 = (long)blockIdx.x * 64;
+  const int t = threadIdx.x, r = t & 63, cg = t >> 6;
+  for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
+    const int rr = idx & 63, k = idx >> 6;
+    Xs[k * LDX + rr] = (k < nb && v0 + rr < nvec) ? B[(v0 + rr) + k * ldb] : T(0);
+    Ys[idx] = inv[idx];
+  }
+  __syncthreads();
+  T acc[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) acc[c] = T(0);
+#pragma unroll 2
+  for (int k = 0; k < nb; k++) {
+    const T xv = Xs[k * LDX + r];
+    const T* yr = Ys + k * TRSM_NB + cg * 16;
+#pragma unroll
+    for (int c = 0; c < 16; c++) acc[c] += xv * yr[c];
+  }
+  if (v0 + r < nvec) {
+#pragma unroll
+    for (int c = 0; c < 16; c++)
+      if (cg * 16 + c < nb) B[(v0 + r) + (long)(cg * 16 + c) * ldb] = acc[c];
+  }
+}
+
+// inverses of op(A)'s diagonal blocks for a right-side solve (a0: the
+// triangle's first element; the recursion splits on the 64-grid from a0)
+template <typename T>
+struct InvCtx {
+  const T* a0 = nullptr;
+  long lda = 0;
+  const T* inv = nullptr;
+};
+
 template <typename T>
 static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nrhs, const T* a, long lda, T* b,
-                 long ldb, hipStream_t s, const int* abort_flag) {
+                 long ldb, hipStream_t s, const int* abort_flag, const InvCtx<T>& ic) {
   if (nb <= 0 || nrhs <= 0) return;
   // E = op(A) for left, op(A)^T for right
   const bool eT = trans != right;
   const bool elower = lower != eT;
+  if (right && ic.inv) {
+    const long blk = (long)(a - ic.a0) / (ic.lda + 1) / TRSM_NB;
+    trsm_right_inv_kernel<T><<<(unsigned)((nrhs + 63) / 64), 256, 0, s>>>(
+        (int)nb, nrhs, ic.inv + blk * TRSM_NB * TRSM_NB, b, ldb, abort_flag);
+    OBA_HIP(hipGetLastError());
+    return;
+  }
   if (right) {
     trsm_leaf_rows_kernel<T><<<(unsigned)((nrhs + 63) / 64), 64, 0, s>>>((int)nb, nrhs, a, lda, b, ldb, eT, elower,
                                                                         unit, abort_flag);
@@ -158,9 +296,9 @@ static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nr
 // alpha already applied; solves in place.
 template <typename T>
 static void rec(bool right, bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b,
-                long ldb, hipStream_t s, const int* abort_flag) {
+                long ldb, hipStream_t s, const int* abort_flag, const InvCtx<T>& ic) {
   const long na = right ? n : m;   // order of the triangle
-  if (na <= TRSM_NB) { leaf<T>(right, lower, trans, unit, na, right ? m : n, a, lda, b, ldb, s, abort_flag); return; }
+  if (na <= TRSM_NB) { leaf<T>(right, lower, trans, unit, na, right ? m : n, a, lda, b, ldb, s, abort_flag, ic); return; }
   long n1 = ((na / 2) + TRSM_NB - 1) / TRSM_NB * TRSM_NB;
   if (n1 >= na) n1 = na - TRSM_NB;
   const long n2 = na - n1;
@@ -173,32 +311,32 @@ static void rec(bool right, bool lower, bool trans, bool unit, long m, long n, c
     T* b1 = b;
     T* b2 = b + n1;
     if (eff_lower) {
-      rec<T>(right, lower, trans, unit, n1, n, a11, lda, b1, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, n1, n, a11, lda, b1, ldb, s, abort_flag, ic);
       // B2 -= op(A)21 * X1 ; op(A)21 = A21 (lower, N) or A12^T (upper, T)
       if (!trans) gemm<T>(false, false, n2, n, n1, T(-1), a21, lda, b1, ldb, T(1), b2, ldb, s, 0, abort_flag);
       else gemm<T>(true, false, n2, n, n1, T(-1), a12, lda, b1, ldb, T(1), b2, ldb, s, 0, abort_flag);
-      rec<T>(right, lower, trans, unit, n2, n, a22, lda, b2, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, n2, n, a22, lda, b2, ldb, s, abort_flag, ic);
     } else {
-      rec<T>(right, lower, trans, unit, n2, n, a22, lda, b2, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, n2, n, a22, lda, b2, ldb, s, abort_flag, ic);
       // B1 -= op(A)12 * X2 ; op(A)12 = A12 (upper, N) or A21^T (lower, T)
       if (!trans) gemm<T>(false, false, n1, n, n2, T(-1), a12, lda, b2, ldb, T(1), b1, ldb, s, 0, abort_flag);
       else gemm<T>(true, false, n1, n, n2, T(-1), a21, lda, b2, ldb, T(1), b1, ldb, s, 0, abort_flag);
-      rec<T>(right, lower, trans, unit, n1, n, a11, lda, b1, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, n1, n, a11, lda, b1, ldb, s, abort_flag, ic);
     }
   } else {
     T* b1 = b;
     T* b2 = b + n1 * ldb;
     if (eff_lower) {
       // X = [X1 X2], op(A) = [A11 0; op21 A22]: X2 first
-      rec<T>(right, lower, trans, unit, m, n2, a22, lda, b2, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, m, n2, a22, lda, b2, ldb, s, abort_flag, ic);
       if (!trans) gemm<T>(false, false, m, n1, n2, T(-1), b2, ldb, a21, lda, T(1), b1, ldb, s, 0, abort_flag);
       else gemm<T>(false, true, m, n1, n2, T(-1), b2, ldb, a12, lda, T(1), b1, ldb, s, 0, abort_flag);
-      rec<T>(right, lower, trans, unit, m, n1, a11, lda, b1, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, m, n1, a11, lda, b1, ldb, s, abort_flag, ic);
     } else {
-      rec<T>(right, lower, trans, unit, m, n1, a11, lda, b1, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, m, n1, a11, lda, b1, ldb, s, abort_flag, ic);
       if (!trans) gemm<T>(false, false, m, n2, n1, T(-1), b1, ldb, a12, lda, T(1), b2, ldb, s, 0, abort_flag);
       else gemm<T>(false, true, m, n2, n1, T(-1), b1, ldb, a21, lda, T(1), b2, ldb, s, 0, abort_flag);
-      rec<T>(right, lower, trans, unit, m, n2, a22, lda, b2, ldb, s, abort_flag);
+      rec<T>(right, lower, trans, unit, m, n2, a22, lda, b2, ldb, s, abort_flag, ic);
     }
   }
 }
@@ -209,7 +347,20 @@ void trsm(bool right, bool lower, bool trans, bool unit, long m, long n, T alpha
   if (m <= 0 || n <= 0) return;
   if (alpha != T(1)) scale<T>(m, n, alpha, b, ldb, s);
   if (alpha == T(0)) return;
-  rec<T>(right, lower, trans, unit, m, n, a, lda, b, ldb, s, abort_flag);
+  InvCtx<T> ic;
+  DevBuf invb;
+  if (right) {
+    // M = op(A): upper when (upper, no transpose) or (lower, transpose)
+    const long nblk = (n + TRSM_NB - 1) / TRSM_NB;
+    invb = DevBuf(sizeof(T) * (size_t)nblk * TRSM_NB * TRSM_NB);
+    trsm_inv_diag_kernel<T><<<(unsigned)nblk, 256, 0, s>>>(n, a, lda, trans, lower == trans, unit, invb.as<T>(),
+                                                           abort_flag);
+    OBA_HIP(hipGetLastError());
+    ic.a0 = a;
+    ic.lda = lda;
+    ic.inv = invb.as<T>();
+  }
+  rec<T>(right, lower, trans, unit, m, n, a, lda, b, ldb, s, abort_flag, ic);
 }
 
 template void trsm<double>(bool, bool, bool, bool, long, long, double, const double*, long, double*, long, hipStream_t, const int*);
