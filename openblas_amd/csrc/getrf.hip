@@ -89,28 +89,32 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // interchange is an O(1) update of two positions and the rows land at their
 // final positions when written back.  Per column: a wavefront max-reduction,
 // one workgroup barrier, the workgroup's best row is published; with one
-// workgroup (P == 1) it goes through LDS, otherwise as a record (position,
-// row number, whole row) written through to memory, followed by one grid
-// barrier (completion wait, atomic arrival counter, spin) executed by the
-// publishing thread alone; after a second workgroup barrier every wavefront reduces the P
-// records itself and copies the pivot row to its own LDS slot.  Two
-// workgroup barriers per column in all.  The column loop runs at run time,
-// but every register index is static: the update step of column j is
+// workgroup (P == 1) it goes through LDS, otherwise the row is written
+// through to memory and, once those stores completed, an 8-byte header
+// {column tag, position}.  Wavefront 0 of every workgroup polls the P
+// headers (one lane each), loads the candidate rows (one more round trip),
+// reduces them and stages the pivot row in LDS; a second workgroup barrier
+// hands it to the other wavefronts.  Two round trips to memory and two
+// workgroup barriers per column; no atomics, no cache maintenance.  The
+// column loop runs at run time, but every register index is static: the
+// update step of column j is
 // selected by a uniform switch (a per-element runtime index costs GPR index
 // mode, measured at ~10 us per column).
-constexpr int LU_NT = 512;            // threads per workgroup
+// threads per workgroup (1024-thread workgroups measured slower: the
+// per-column barriers cost more with 16 wavefronts)
+template <typename T> struct LuNt { static constexpr int v = 512; };
 constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
 
 // rows per thread (registers: RPT * W values of T per thread)
 template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
 
-// candidate records of one column: [2 buffers][LU_PMAX] positions, row
-// numbers and rows (double-buffered by column parity: a workgroup can only
-// publish column j+2 after every workgroup has read column j)
+// candidate records of one column: [2 buffers][LU_PMAX] headers and rows
+// (double-buffered by column parity: a workgroup can only publish column
+// j+2 after every workgroup has read column j+1, hence column j).  The tag
+// base + j + 1 never repeats within one factorization.
 template <typename T>
 struct PanelRec {
-  int* pos;
-  int* phys;
+  unsigned long long* hdr;   // {tag (low 32 bits), position (high 32 bits)}
   T* row;
 };
 
@@ -159,8 +163,8 @@ __device__ __attribute__((always_inline)) inline void lu_col_dispatch(int j, F&&
 template <typename T, int W, int RPT, int NT>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
                                                       int* __restrict__ ipiv, int* __restrict__ info,
-                                                      int* __restrict__ moves, PanelRec<T> rec, unsigned* counter,
-                                                      unsigned base, unsigned* err) {
+                                                      int* __restrict__ moves, PanelRec<T> rec, unsigned base,
+                                                      unsigned* err) {
   static_assert(W <= 64, "one pivot-row element per lane");
   constexpr int ROWS = NT * RPT;
   constexpr int NWAVE = NT / 64;
@@ -168,8 +172,8 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   const long mp = m - j0;
   __shared__ double s_v[NWAVE];
   __shared__ int s_p[NWAVE];
-  __shared__ T s_row[W];            // P == 1: the pivot row
-  __shared__ T s_wrow[NWAVE][W];    // P > 1: each wavefront's copy of the pivot row
+  __shared__ T s_row[W];            // the pivot row
+  __shared__ int s_gp;              // its position
   __shared__ int s_bad;
 
   T x[RPT][W];
@@ -229,59 +233,82 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       gp = wp;
       urow = s_row;
     } else {
-      // (c) publish the candidate record; the publishing thread alone runs
-      // the grid barrier (release, arrive, bounded spin, acquire)
+      // (c) publish: the workgroup's best row with write-through stores,
+      // then (after they complete) an 8-byte header {tag, position}
       const int slot = buf * LU_PMAX + g;
+      const unsigned tag = base + (unsigned)(j + 1);
       bool pub = false;
+      int hpos = INT_MAX;
 #pragma unroll
       for (int q = 0; q < RPT; q++)
         if (live[q] && pos[q] == wp) {
-          wt_store(&rec.pos[slot], wp);
-          wt_store(&rec.phys[slot], g * ROWS + q * NT + t);
 #pragma unroll
           for (int c = 0; c < W; c++) wt_store(&rec.row[(size_t)slot * W + c], x[q][c]);
           pub = true;
+          hpos = wp;
         }
-      if (wp == INT_MAX && t == 0) {   // no live row left in this workgroup
-        wt_store(&rec.pos[slot], INT_MAX);
-        wt_store(&rec.phys[slot], 0);
-        pub = true;
-      }
+      if (wp == INT_MAX && t == 0) pub = true;   // no live row left in this workgroup
       if (pub) {
-        // the record went out with write-through stores: wait for them to
-        // complete, then arrive (no L2 writeback needed)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        atomicAdd(counter, 1u);
-        const unsigned target = base + (unsigned)P * (unsigned)(j + 1);
-        long spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-          if (++spins > (1L << 22)) { s_bad = 1; atomicExch(err, 1u); break; }
+        wt_store(&rec.hdr[slot], ((unsigned long long)(unsigned)hpos << 32) | tag);
+      }
+      // (d) wavefront 0 waits for every header of this column, reduces the
+      // candidates and stages the pivot row in LDS for the workgroup
+      if (wave == 0) {
+        double v = -2.0;
+        int p = INT_MAX, wi = -1;
+        bool ok = true;
+        T rowv[W];
+        for (int c0 = 0; c0 < P; c0 += 64) {
+          const int c = c0 + lane;
+          if (c < P) {
+            const unsigned long long* hp = &rec.hdr[buf * LU_PMAX + c];
+            unsigned long long h;
+            long spins = 0;
+            while (((h = bypass_load(hp)) & 0xffffffffull) != tag) {
+              __builtin_amdgcn_s_sleep(1);
+              if (++spins > (1L << 22)) { ok = false; break; }
+            }
+            const int cp = (int)(h >> 32);
+            if (ok && cp != INT_MAX) {
+              const T* rp = &rec.row[(size_t)(buf * LU_PMAX + c) * W];
+              T key = T(0);
+              if (P <= 64) {   // the whole row comes with this round trip
+#pragma unroll
+                for (int e = 0; e < W; e++) rowv[e] = bypass_load(rp + e);
+#pragma unroll
+                for (int e = 0; e < W; e++) key = (e == j) ? rowv[e] : key;
+              } else {
+                key = bypass_load(rp + j);
+              }
+              const double vv = pivot_key(key, cp, j);
+              if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
+            }
+          }
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(v, off);
+          const int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
+          if (better(ov, op, v, p)) { v = ov; p = op; wi = ow; }
+        }
+        const bool allok = __all(ok);
+        if (!allok || wi < 0 || wi >= P || p < j || p >= mp) {   // timeout; the rest cannot happen
+          if (lane == 0) { s_bad = 1; atomicExch(err, allok ? 2u : 1u); }
+        } else if (P <= 64) {
+          if (lane == wi) {
+#pragma unroll
+            for (int e = 0; e < W; e++) s_row[e] = rowv[e];
+          }
+        } else if (lane < W) {
+          s_row[lane] = bypass_load(&rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane]);
+        }
+        if (lane == 0) s_gp = p;
       }
       __syncthreads();
       if (s_bad) break;
-      // (d) every wavefront reduces the P records on its own
-      double v = -2.0;
-      int p = INT_MAX, wi = -1;
-      for (int c = lane; c < P; c += 64) {
-        const int cp = bypass_load(&rec.pos[buf * LU_PMAX + c]);
-        if (cp == INT_MAX) continue;
-        const double vv = pivot_key(bypass_load(&rec.row[(size_t)(buf * LU_PMAX + c) * W + j]), cp, j);
-        if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
-        if (better(ov, op, v, p)) { v = ov; p = op; wi = ow; }
-      }
-      if (wi < 0 || wi >= P || p < j || p >= mp) {   // cannot happen: mp > j rows are live
-        if (t == 0) atomicExch(err, 2u);
-        break;
-      }
-      gp = p;
-      if (lane < W) s_wrow[wave][lane] = bypass_load(&rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane]);
-      urow = s_wrow[wave];   // written and read by this wavefront only
+      gp = s_gp;
+      urow = s_row;
     }
     const T piv = urow[j];
     const bool nz = piv != T(0);
@@ -376,7 +403,6 @@ struct LuCtx {
   int* moves;          // LU_MV ints per panel: a count and up to 2*LU_W pairs
   long panel_no = 0;
   PanelRec<T> rec;
-  unsigned* counter;
   unsigned base = 0;
   unsigned* err;
   hipStream_t s;
@@ -386,14 +412,15 @@ template <typename T>
 static void lu_panel(LuCtx<T>& c, long j0, int w) {
   const long mp = c.m - j0;
   constexpr int RPT = LuRpt<T>::v;
-  constexpr long ROWS = (long)LU_NT * RPT;
+  constexpr int NT = LuNt<T>::v;
+  constexpr long ROWS = (long)NT * RPT;
   long P = (mp + ROWS - 1) / ROWS;
   if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
   int* mv = c.moves + c.panel_no * LU_MV;
-  lu_panel_kernel<T, LU_W, RPT, LU_NT><<<(unsigned)P, LU_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
-                                                                        c.rec, c.counter, c.base, c.err);
+  lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
+                                                                        c.rec, c.base, c.err);
   OBA_HIP(hipGetLastError());
-  if (P > 1) c.base += (unsigned)P * (unsigned)w;   // one arrival per workgroup and column
+  if (P > 1) c.base += (unsigned)w;   // header tags of this panel: base+1 .. base+w
   c.panel_no++;
   if (c.nrows_t - w > 0) {
     lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 63) / 64), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);
@@ -434,10 +461,8 @@ int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hip
   c.At = At; c.ldt = ldt; c.m = m; c.nrows_t = n + (track_perm ? 1 : 0); c.ipiv = ipiv_dev; c.s = s;
   c.info = (int*)p;
   c.err = (unsigned*)(p + 64);
-  c.counter = (unsigned*)(p + 128);
   c.rec.row = (T*)(p + 4096);
-  c.rec.pos = (int*)(p + 4096 + sizeof(T) * LU_W * 2 * LU_PMAX);
-  c.rec.phys = c.rec.pos + 2 * LU_PMAX;
+  c.rec.hdr = (unsigned long long*)(p + 4096 + sizeof(T) * LU_W * 2 * LU_PMAX);
   c.moves = (int*)(p + 4096 + rec_bytes);
   OBA_HIP(hipMemsetAsync(p, 0, 4096 + rec_bytes + mv_bytes, s));
   if (track_perm) {
