@@ -302,13 +302,18 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
   const bool vec = (lda % V == 0) && (ldb % V == 0) && (((uintptr_t)a & 15) == 0) && (((uintptr_t)b & 15) == 0);
   const long big_tiles = ((m + 127) / 128) * ((n + 127) / 128);
   const bool small = big_tiles < rt().num_cu;
-  constexpr int BKB = sizeof(T) == 8 ? 16 : 32;
+  // k-slab depths from a sweep on MI355X (8192^3): shallow slabs keep the
+  // LDS footprint small and the MFMA pipe fed (FP64 128x128: BK 16 -> 4 is
+  // 60.6 -> 66.0 TF/s; FP32 BK 32 -> 8 is 118.5 -> 134.8 TF/s); the tile
+  // loaders need TILE*BK >= 256 vectors of 16 bytes
+  constexpr int BK_BIG = sizeof(T) == 8 ? 4 : 8;
+  constexpr int BK_SMALL = sizeof(T) == 8 ? 8 : 16;
   if (small) {
-    if (vec) launch_cfg<T, 64, 64, BKB, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
-    else launch_cfg<T, 64, 64, BKB, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    if (vec) launch_cfg<T, 64, 64, BK_SMALL, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    else launch_cfg<T, 64, 64, BK_SMALL, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
   } else {
-    if (vec) launch_cfg<T, 128, 128, BKB, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
-    else launch_cfg<T, 128, 128, BKB, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    if (vec) launch_cfg<T, 128, 128, BK_BIG, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
+    else launch_cfg<T, 128, 128, BK_BIG, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
   }
 }
 
