@@ -27,12 +27,18 @@ struct Runtime {
   int device = -1;
   int num_cu = 0;
   hipStream_t stream = nullptr;   // all library work is ordered on this
-  hipStream_t side = nullptr;     // look-ahead panel work (same priority)
+  hipStream_t side = nullptr;     // second stream (same priority)
+  hipStream_t bulk = nullptr;     // like side, restricted to all but a few CUs
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   bool sync_on_return = true;     // synchronous semantics like the reference
   std::recursive_mutex lock;      // library calls are serialized
 };
 Runtime& rt();                    // initializes on first use
+// Stream for throughput-bound work that runs beside latency-bound work on
+// rt().stream: its kernels use all CUs but OPENBLAS_AMD_RESERVED_CUS (32 by
+// default), which stay free for the other stream (falls back to rt().side
+// when the device refuses CU masks).
+hipStream_t bulk_stream();
 
 // Device scratch pool (the analogue of blas_memory_alloc,
 // driver/others/memory.c:1161): cached blocks, reused across calls.

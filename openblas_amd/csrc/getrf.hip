@@ -32,6 +32,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -87,16 +88,19 @@ __global__ void iota_row_kernel(T* row, long n, long stride) {
 // Rows are never moved inside the kernel: each keeps its current position
 // (LAPACK order after the interchanges so far) in a register, so an
 // interchange is an O(1) update of two positions and the rows land at their
-// final positions when written back.  Per column: a wavefront max-reduction,
-// one workgroup barrier, the workgroup's best row is published; with one
-// workgroup (P == 1) it goes through LDS, otherwise the row is written
-// through to memory and, once those stores completed, an 8-byte header
-// {column tag, position}.  Wavefront 0 of every workgroup polls the P
-// headers (one lane each), loads the candidate rows (one more round trip),
-// reduces them and stages the pivot row in LDS; a second workgroup barrier
-// hands it to the other wavefronts.  Two round trips to memory and two
-// workgroup barriers per column; no atomics, no cache maintenance.  The
-// column loop runs at run time, but every register index is static: the
+// final positions when written back.  Per column: a wavefront max-reduction
+// (each wavefront's best row staged in LDS), one workgroup barrier, the
+// workgroup's best row is published; with one workgroup (P == 1) it goes
+// through LDS, otherwise wavefront 0 writes it through to memory as a
+// record of 8-byte words, each word = {32 bits of the row, column tag}, plus
+// one word {position, tag}.  An 8-byte store is seen whole or not at all, so
+// a record is complete exactly when every word carries the tag: no store
+// completion wait, no separate header, and the readers (all wavefronts, a
+// share of the records each, coalesced loads) get the rows in the same round
+// trip that shows they are there.  A second workgroup barrier hands the
+// loaded rows and the per-wavefront winners to every thread.  No atomics, no
+// cache maintenance.  The column loop runs at run time, but every register
+// index is static: the
 // update step of column j is
 // selected by a uniform switch (a per-element runtime index costs GPR index
 // mode, measured at ~10 us per column).
@@ -108,14 +112,16 @@ constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pair
 // rows per thread (registers: RPT * W values of T per thread)
 template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
 
-// candidate records of one column: [2 buffers][LU_PMAX] headers and rows
-// (double-buffered by column parity: a workgroup can only publish column
-// j+2 after every workgroup has read column j+1, hence column j).  The tag
-// base + j + 1 never repeats within one factorization.
-template <typename T>
-struct PanelRec {
-  unsigned long long* hdr;   // {tag (low 32 bits), position (high 32 bits)}
-  T* row;
+// candidate records of one column: [2 buffers][LU_PMAX] records of
+// LuRec::RW words (double-buffered by column parity: a workgroup can only
+// publish column j+2 after every workgroup has read column j+1, hence
+// column j).  The tag base + j + 1 never repeats within one factorization
+// and the records start zeroed.
+template <typename T, int W>
+struct LuRec {
+  static constexpr int DW = W * (int)sizeof(T) / 4;   // 32-bit words of a row
+  static constexpr int RW = DW + 1;                   // + the position word
+  static_assert(DW <= 64 && 64 % DW == 0, "a load instruction covers whole records");
 };
 
 // Cross-workgroup records: write-through stores (to memory, past the
@@ -163,18 +169,23 @@ __device__ __attribute__((always_inline)) inline void lu_col_dispatch(int j, F&&
 template <typename T, int W, int RPT, int NT>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
                                                       int* __restrict__ ipiv, int* __restrict__ info,
-                                                      int* __restrict__ moves, PanelRec<T> rec, unsigned base,
-                                                      unsigned* err) {
+                                                      int* __restrict__ moves, unsigned long long* __restrict__ rec,
+                                                      unsigned base, unsigned* err) {
   static_assert(W <= 64, "one pivot-row element per lane");
   constexpr int ROWS = NT * RPT;
   constexpr int NWAVE = NT / 64;
   const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const long mp = m - j0;
+  constexpr int DW = LuRec<T, W>::DW, RW = LuRec<T, W>::RW;
+  constexpr int RPI = 64 / DW;      // records per load instruction
   __shared__ double s_v[NWAVE];
   __shared__ int s_p[NWAVE];
-  __shared__ T s_row[W];            // the pivot row
-  __shared__ int s_gp;              // its position
-  __shared__ int s_bad;
+  __shared__ T s_row[W];            // the pivot row (P == 1)
+  __shared__ T s_cand[NWAVE][W];    // each wavefront's best row
+  __shared__ T s_rows[LU_PMAX][W];  // the P candidate rows of the column
+  __shared__ double s_wv[NWAVE];    // per wavefront: best record key,
+  __shared__ int s_wp[NWAVE];       // its position,
+  __shared__ int s_wr[NWAVE];       // its record (-2: timed out)
 
   T x[RPT][W];
   T cand[RPT];      // element j of each row: the pivot candidates of column j
@@ -190,8 +201,6 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
     cand[q] = x[q][0];
   }
-  if (t == 0) s_bad = 0;
-  __syncthreads();
 
 #pragma unroll 1
   for (int j = 0; j < w; j++) {
@@ -212,14 +221,22 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       const int op = __shfl_xor(bp, off);
       if (better(ov, op, bv, bp)) { bv = ov; bp = op; }
     }
+    if (P > 1) {   // the wavefront's best row, for the publishing wavefront
+#pragma unroll
+      for (int q = 0; q < RPT; q++)
+        if (live[q] && pos[q] == bp) {
+#pragma unroll
+          for (int c = 0; c < W; c++) s_cand[wave][c] = x[q][c];
+        }
+    }
     if (lane == 0) { s_v[wave] = bv; s_p[wave] = bp; }
     __syncthreads();
     // (b) the workgroup's best (every thread reduces the same values)
     double wv = s_v[0];
-    int wp = s_p[0];
+    int wp = s_p[0], wk = 0;
 #pragma unroll
     for (int k = 1; k < NWAVE; k++)
-      if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; }
+      if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; wk = k; }
     int gp;          // pivot position
     const T* urow;   // the pivot row in LDS
     if (P == 1) {
@@ -233,82 +250,79 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       gp = wp;
       urow = s_row;
     } else {
-      // (c) publish: the workgroup's best row with write-through stores,
-      // then (after they complete) an 8-byte header {tag, position}
-      const int slot = buf * LU_PMAX + g;
       const unsigned tag = base + (unsigned)(j + 1);
-      bool pub = false;
-      int hpos = INT_MAX;
-#pragma unroll
-      for (int q = 0; q < RPT; q++)
-        if (live[q] && pos[q] == wp) {
-#pragma unroll
-          for (int c = 0; c < W; c++) wt_store(&rec.row[(size_t)slot * W + c], x[q][c]);
-          pub = true;
-          hpos = wp;
-        }
-      if (wp == INT_MAX && t == 0) pub = true;   // no live row left in this workgroup
-      if (pub) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        wt_store(&rec.hdr[slot], ((unsigned long long)(unsigned)hpos << 32) | tag);
-      }
-      // (d) wavefront 0 waits for every header of this column, reduces the
-      // candidates and stages the pivot row in LDS for the workgroup
+      const unsigned long long tg = (unsigned long long)tag << 32;
+      unsigned long long* recs = rec + (size_t)buf * LU_PMAX * RW;
+      // (c) publish: word e of the record = {word e of the row, tag}; the
+      // last word {position (INT_MAX: no live row here), tag}
       if (wave == 0) {
-        double v = -2.0;
-        int p = INT_MAX, wi = -1;
-        bool ok = true;
-        T rowv[W];
-        for (int c0 = 0; c0 < P; c0 += 64) {
-          const int c = c0 + lane;
-          if (c < P) {
-            const unsigned long long* hp = &rec.hdr[buf * LU_PMAX + c];
-            unsigned long long h;
-            long spins = 0;
-            while (((h = bypass_load(hp)) & 0xffffffffull) != tag) {
-              __builtin_amdgcn_s_sleep(1);
-              if (++spins > (1L << 22)) { ok = false; break; }
-            }
-            const int cp = (int)(h >> 32);
-            if (ok && cp != INT_MAX) {
-              const T* rp = &rec.row[(size_t)(buf * LU_PMAX + c) * W];
-              T key = T(0);
-              if (P <= 64) {   // the whole row comes with this round trip
-#pragma unroll
-                for (int e = 0; e < W; e++) rowv[e] = bypass_load(rp + e);
-#pragma unroll
-                for (int e = 0; e < W; e++) key = (e == j) ? rowv[e] : key;
-              } else {
-                key = bypass_load(rp + j);
-              }
-              const double vv = pivot_key(key, cp, j);
-              if (better(vv, cp, v, p)) { v = vv; p = cp; wi = c; }
-            }
-          }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(v, off);
-          const int op = __shfl_xor(p, off), ow = __shfl_xor(wi, off);
-          if (better(ov, op, v, p)) { v = ov; p = op; wi = ow; }
-        }
-        const bool allok = __all(ok);
-        if (!allok || wi < 0 || wi >= P || p < j || p >= mp) {   // timeout; the rest cannot happen
-          if (lane == 0) { s_bad = 1; atomicExch(err, allok ? 2u : 1u); }
-        } else if (P <= 64) {
-          if (lane == wi) {
-#pragma unroll
-            for (int e = 0; e < W; e++) s_row[e] = rowv[e];
-          }
-        } else if (lane < W) {
-          s_row[lane] = bypass_load(&rec.row[(size_t)(buf * LU_PMAX + wi) * W + lane]);
-        }
-        if (lane == 0) s_gp = p;
+        unsigned long long* dst = recs + (size_t)g * RW;
+        const unsigned* srow = reinterpret_cast<const unsigned*>(&s_cand[wk][0]);
+        if (lane < DW) wt_store(dst + lane, tg | srow[lane]);
+        if (lane == 0) wt_store(dst + DW, tg | (unsigned)wp);
       }
+      // (d) wavefront v collects records v*RPI.., (v+NWAVE)*RPI.., ...:
+      // lane = h*DW + d loads word d of record r0+h, and word DW when d == 0
+      const int h = lane / DW, d = lane % DW;
+      double bv2 = -2.0;
+      int bp2 = INT_MAX, br2 = -1;
+      bool ok = true;
+      for (int r0 = wave * RPI; r0 < P; r0 += NWAVE * RPI) {
+        const int r = r0 + h;
+        const unsigned long long* src = recs + (size_t)r * RW;
+        unsigned long long wd = 0, wq = 0;
+        bool have = r >= P;
+        long spins = 0;
+        while (true) {
+          if (!have) {
+            wd = bypass_load(src + d);
+            if (d == 0) wq = bypass_load(src + DW);
+            have = (unsigned)(wd >> 32) == tag && (d != 0 || (unsigned)(wq >> 32) == tag);
+          }
+          if (__all(have)) break;
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > (1L << 22)) { ok = false; break; }
+        }
+        if (!ok) break;
+        if (r < P) reinterpret_cast<unsigned*>(&s_rows[r][0])[d] = (unsigned)wd;
+        const int wdi = (int)(unsigned)wd, wqi = (int)(unsigned)wq;
+#pragma unroll
+        for (int hh = 0; hh < RPI; hh++) {
+          if (r0 + hh < P) {
+            const int cp = __builtin_amdgcn_readlane(wqi, hh * DW);
+            T key;
+            if constexpr (sizeof(T) == 8) {
+              const int lo = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * j);
+              const int hi = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * j + 1);
+              key = __hiloint2double(hi, lo);
+            } else {
+              key = __int_as_float(__builtin_amdgcn_readlane(wdi, hh * DW + j));
+            }
+            if (cp != INT_MAX) {
+              const double vv = pivot_key(key, cp, j);
+              if (better(vv, cp, bv2, bp2)) { bv2 = vv; bp2 = cp; br2 = r0 + hh; }
+            }
+          }
+        }
+      }
+      if (lane == 0) { s_wv[wave] = bv2; s_wp[wave] = bp2; s_wr[wave] = ok ? br2 : -2; }
       __syncthreads();
-      if (s_bad) break;
-      gp = s_gp;
-      urow = s_row;
+      // (e) the column's pivot: every thread reduces the same values
+      double fv = s_wv[0];
+      int fp = s_wp[0], fr = s_wr[0];
+      bool allok = fr != -2;
+#pragma unroll
+      for (int k = 1; k < NWAVE; k++) {
+        const int kr = s_wr[k];
+        allok = allok && kr != -2;
+        if (better(s_wv[k], s_wp[k], fv, fp)) { fv = s_wv[k]; fp = s_wp[k]; fr = kr; }
+      }
+      if (!allok || fr < 0 || fr >= P || fp < j || fp >= mp) {   // timeout; the rest cannot happen
+        if (t == 0) atomicExch(err, allok ? 2u : 1u);
+        break;
+      }
+      gp = fp;
+      urow = &s_rows[fr][0];
     }
     const T piv = urow[j];
     const bool nz = piv != T(0);
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       lu_col_step<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow, cand);
     });
   }
-  if (s_bad || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   // every row to its final position; list the rows that moved
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
@@ -356,14 +370,15 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   }
 }
 
-// Apply the panel's interchanges to every other column of A, i.e. to the
-// rows [0, nrows_t) \ [j0, j0+w) of At: the moves (destination position,
-// source row) form a permutation of at most 2w columns of At, so every
-// source element is read before any destination is written (no sequential
-// dependence, coalesced along the rows of At).
+// Apply a panel's interchanges to the columns of A in [lo, hi) outside
+// [ex_lo, ex_hi), i.e. to those rows of At: the moves (destination position,
+// source row, both relative to the panel's first row j0) form a permutation
+// of at most 2w columns of At, so every source element is read before any
+// destination is written (no sequential dependence, coalesced along the rows
+// of At).  The grid covers [lo, hi) in 64-row slabs.
 template <typename T, int W>
-__global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long ldt, long nrows_t, long j0, int w,
-                                                      long mp, const int* __restrict__ moves) {
+__global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long ldt, long lo, long hi, long ex_lo,
+                                                      long ex_hi, long j0, long mp, const int* __restrict__ moves) {
   constexpr int RB = 64;                       // At rows per workgroup
   __shared__ int sd[2 * W], ss[2 * W];
   __shared__ int s_cnt;
@@ -378,21 +393,35 @@ __global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long l
     ss[threadIdx.x] = sr;
   }
   __syncthreads();
-  const long r0 = (long)blockIdx.x * RB;
+  const long r0 = lo + (long)blockIdx.x * RB;
   // gather every source first (coalesced along the rows of At) ...
   for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
     const int k = idx / RB, rr = idx % RB;
     const long r = r0 + rr;
-    if (r < nrows_t && !(r >= j0 && r < j0 + w)) vals[k][rr] = At[r + (j0 + ss[k]) * ldt];
+    if (r < hi && !(r >= ex_lo && r < ex_hi)) vals[k][rr] = At[r + (j0 + ss[k]) * ldt];
   }
   __syncthreads();
   // ... then scatter: the moves are a permutation of the touched columns
   for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
     const int k = idx / RB, rr = idx % RB;
     const long r = r0 + rr;
-    if (r < nrows_t && !(r >= j0 && r < j0 + w)) At[r + (j0 + sd[k]) * ldt] = vals[k][rr];
+    if (r < hi && !(r >= ex_lo && r < ex_hi)) At[r + (j0 + sd[k]) * ldt] = vals[k][rr];
   }
 }
+
+template <typename T>
+static void lu_swap(T* At, long ldt, long lo, long hi, long ex_lo, long ex_hi, long j0, long mp, const int* mv,
+                    hipStream_t s) {
+  if (hi <= lo || (lo >= ex_lo && hi <= ex_hi)) return;   // nothing outside the excluded range
+  lu_swap_kernel<T, LU_W><<<(unsigned)((hi - lo + 63) / 64), 256, 0, s>>>(At, ldt, lo, hi, ex_lo, ex_hi, j0, mp, mv);
+  OBA_HIP(hipGetLastError());
+}
+
+// one factored leaf panel: its first row/column and its moves slot
+struct LuLeaf {
+  long j0;
+  int* mv;
+};
 
 template <typename T>
 struct LuCtx {
@@ -402,9 +431,11 @@ struct LuCtx {
   int* info;
   int* moves;          // LU_MV ints per panel: a count and up to 2*LU_W pairs
   long panel_no = 0;
-  PanelRec<T> rec;
+  unsigned long long* rec;   // candidate records (LuRec)
   unsigned base = 0;
   unsigned* err;
+  long sw_lo = 0, sw_hi = 0;     // At rows (columns of A) the leaf interchanges go to at once
+  std::vector<LuLeaf> leaves;    // every leaf so far, in order
   hipStream_t s;
 };
 
@@ -420,12 +451,10 @@ static void lu_panel(LuCtx<T>& c, long j0, int w) {
   lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
                                                                         c.rec, c.base, c.err);
   OBA_HIP(hipGetLastError());
-  if (P > 1) c.base += (unsigned)w;   // header tags of this panel: base+1 .. base+w
+  if (P > 1) c.base += (unsigned)w;   // record tags of this panel: base+1 .. base+w
   c.panel_no++;
-  if (c.nrows_t - w > 0) {
-    lu_swap_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 63) / 64), 256, 0, c.s>>>(c.At, c.ldt, c.nrows_t, j0, w, mp, mv);
-    OBA_HIP(hipGetLastError());
-  }
+  c.leaves.push_back(LuLeaf{j0, mv});
+  lu_swap<T>(c.At, c.ldt, c.sw_lo, c.sw_hi, j0, j0 + w, j0, mp, mv, c.s);
 }
 
 template <typename T>
@@ -448,12 +477,81 @@ static void lu_rec(LuCtx<T>& c, long j0, long nc) {
   lu_rec<T>(c, j0 + n1, n2);
 }
 
+// Blocked right-looking LU with look-ahead, for large factorizations: the
+// columns are cut into blocks of nb; block k+1 is factored (recursively, as
+// above, its interchanges kept inside the block) on the caller's stream while
+// the trailing update by block k -- the bulk of the flops, one TRSM and one
+// large GEMM -- runs on a second stream that leaves some CUs free, so the
+// latency-bound leaf panels and the throughput-bound GEMM overlap.  A
+// block's interchanges reach the columns outside it after the trailing update
+// that reads them has finished.  The trailing updates are sums over the
+// blocks instead of over the recursion's halves: a different association of
+// the same products (rounding-level differences, as with any blocking).
+template <typename T>
+static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
+  Runtime& r = rt();
+  hipStream_t M = c.s, X = bulk_stream();
+  T* At = c.At;
+  const long ldt = c.ldt;
+  DevBuf invb[2] = {DevBuf(trsm_right_inv_bytes<T>(nb)), DevBuf(trsm_right_inv_bytes<T>(nb))};
+  auto factor_block = [&](long jb, long w) {
+    c.sw_lo = jb;
+    c.sw_hi = jb + w;
+    const size_t l0 = c.leaves.size();
+    lu_rec<T>(c, jb, w);
+    return l0;
+  };
+  size_t l0 = factor_block(0, std::min(nb, mn));
+  bool pending = false;
+  for (long jb = 0; jb < mn; jb += nb) {
+    const long w = std::min(nb, mn - jb), jn = jb + w;
+    if (pending) {   // the trailing update by the previous block has finished
+      OBA_HIP(hipStreamWaitEvent(M, r.ev_b, 0));
+      pending = false;
+    }
+    for (size_t l = l0; l < c.leaves.size(); l++)
+      lu_swap<T>(At, ldt, 0, c.nrows_t, jb, jn, c.leaves[l].j0, c.m - c.leaves[l].j0, c.leaves[l].mv, M);
+    if (jn >= mn) break;
+    const long w1 = std::min(nb, mn - jn);
+    const long below = c.m - jn;
+    T* inv = invb[(jb / nb) & 1].as<T>();
+    const T* u = At + jb + jb * ldt;   // unit upper triangle of the block (At view)
+    trsm_right_prepare<T>(false, false, true, w, u, ldt, inv, M);
+    // next block: At rows [jn, jn+w1)
+    trsm_right_prepared<T>(false, false, true, w1, w, u, ldt, At + jn + jb * ldt, ldt, inv, M);
+    gemm<T>(false, false, w1, below, w, T(-1), At + jn + jb * ldt, ldt, At + jb + jn * ldt, ldt, T(1),
+            At + jn + jn * ldt, ldt, M);
+    // trailing columns [jn+w1, mn) on the bulk stream
+    const long rest = mn - (jn + w1);
+    if (rest > 0) {
+      OBA_HIP(hipEventRecord(r.ev_a, M));
+      OBA_HIP(hipStreamWaitEvent(X, r.ev_a, 0));
+      T* xr = At + (jn + w1) + jb * ldt;
+      trsm_right_prepared<T>(false, false, true, rest, w, u, ldt, xr, ldt, inv, X);
+      gemm<T>(false, false, rest, below, w, T(-1), xr, ldt, At + jb + jn * ldt, ldt, T(1), At + (jn + w1) + jn * ldt,
+              ldt, X);
+      OBA_HIP(hipEventRecord(r.ev_b, X));
+      pending = true;
+    }
+    l0 = factor_block(jn, w1);
+  }
+  if (pending) OBA_HIP(hipStreamWaitEvent(M, r.ev_b, 0));
+}
+
+// block width of the look-ahead driver (OPENBLAS_AMD_LU_NB; 0 turns it off);
+// it is used from 4 blocks on
+static long lu_block_nb() {
+  long v = 256;   // swept at n = 16384 over 128..1024 (with 16..64 CUs kept free)
+  if (const char* e = getenv("OPENBLAS_AMD_LU_NB")) v = atol(e);
+  return v;
+}
+
 template <typename T>
 int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hipStream_t s) {
   const long mn = std::min(m, n);
   if (mn <= 0) return 0;
   const long npanels = (mn + LU_W - 1) / LU_W + 1;
-  const size_t rec_bytes = (sizeof(int) * 2 + sizeof(T) * LU_W) * 2 * LU_PMAX;
+  const size_t rec_bytes = sizeof(unsigned long long) * LuRec<T, LU_W>::RW * 2 * LU_PMAX;
   const size_t mv_bytes = sizeof(int) * npanels * LU_MV;
   DevBuf ws(4096 + rec_bytes + mv_bytes);
   char* p = ws.as<char>();
@@ -461,15 +559,21 @@ int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hip
   c.At = At; c.ldt = ldt; c.m = m; c.nrows_t = n + (track_perm ? 1 : 0); c.ipiv = ipiv_dev; c.s = s;
   c.info = (int*)p;
   c.err = (unsigned*)(p + 64);
-  c.rec.row = (T*)(p + 4096);
-  c.rec.hdr = (unsigned long long*)(p + 4096 + sizeof(T) * LU_W * 2 * LU_PMAX);
+  c.rec = (unsigned long long*)(p + 4096);
   c.moves = (int*)(p + 4096 + rec_bytes);
   OBA_HIP(hipMemsetAsync(p, 0, 4096 + rec_bytes + mv_bytes, s));
   if (track_perm) {
     iota_row_kernel<T><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(At + n, m, ldt);
     OBA_HIP(hipGetLastError());
   }
-  lu_rec<T>(c, 0, mn);
+  const long nb = lu_block_nb();
+  if (nb >= 2 * LU_W && nb % LU_W == 0 && mn >= 4 * nb) {
+    lu_blocked<T>(c, mn, nb);
+  } else {
+    c.sw_lo = 0;
+    c.sw_hi = c.nrows_t;
+    lu_rec<T>(c, 0, mn);
+  }
   if (n > mn) trsm<T>(true, false, false, true, n - mn, mn, T(1), At, ldt, At + mn, ldt, s);
   int h[2] = {0, 0};
   OBA_HIP(hipMemcpyAsync(&h[0], c.info, sizeof(int), hipMemcpyDeviceToHost, s));

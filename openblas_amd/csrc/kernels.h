@@ -27,6 +27,18 @@ template <typename T>
 void trsm(bool right, bool lower, bool trans, bool unit, long m, long n, T alpha, const T* a,
           long lda, T* b, long ldb, hipStream_t s, const int* abort_flag = nullptr);
 
+// Right-side solve X <- X * op(A)^{-1} (alpha = 1) in two steps, so that the
+// inverses of op(A)'s 64x64 diagonal blocks can be computed once and reused
+// by several solves with the same triangle (on any stream, no allocation):
+// trsm_right_inv_bytes(n) bytes for `inv`.
+template <typename T>
+size_t trsm_right_inv_bytes(long n);
+template <typename T>
+void trsm_right_prepare(bool lower, bool trans, bool unit, long n, const T* a, long lda, T* inv, hipStream_t s);
+template <typename T>
+void trsm_right_prepared(bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b, long ldb,
+                         const T* inv, hipStream_t s);
+
 // LU with partial pivoting, in place; ipiv is a 1-based device int array.
 // Returns info (0, or first zero pivot j>0) -- lapack/getrf/getrf_single.c.
 template <typename T>

@@ -64,6 +64,36 @@ Runtime& rt() {
   return *g_rt;
 }
 
+hipStream_t bulk_stream() {
+  Runtime& r = rt();
+  static std::once_flag once;
+  std::call_once(once, [&r] {
+    int reserve = 32;
+    if (const char* e = getenv("OPENBLAS_AMD_RESERVED_CUS")) reserve = atoi(e);
+    if (reserve > 0 && r.num_cu > 2 * reserve) {
+      std::vector<uint32_t> mask((size_t)(r.num_cu + 31) / 32, 0u);
+      for (int i = 0; i < r.num_cu - reserve; i++) mask[(size_t)i / 32] |= 1u << (i % 32);
+      hipStream_t st = nullptr;
+      if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) == hipSuccess) r.bulk = st;
+      else (void)hipGetLastError();
+    }
+    if (r.bulk) {
+      // drain and release the masked queue before the runtime tears down
+      std::atexit([] {
+        Runtime& q = *g_rt;
+        if (q.bulk && q.bulk != q.side) {
+          (void)hipStreamSynchronize(q.bulk);
+          (void)hipStreamDestroy(q.bulk);
+          q.bulk = q.side;
+        }
+      });
+    } else {
+      r.bulk = r.side;
+    }
+  });
+  return r.bulk;
+}
+
 void end_call() {
   Runtime& r = rt();
   if (r.sync_on_return) OBA_HIP(hipStreamSynchronize(r.stream));

@@ -363,6 +363,38 @@ void trsm(bool right, bool lower, bool trans, bool unit, long m, long n, T alpha
   rec<T>(right, lower, trans, unit, m, n, a, lda, b, ldb, s, abort_flag, ic);
 }
 
+template <typename T>
+size_t trsm_right_inv_bytes(long n) {
+  return sizeof(T) * (size_t)((n + TRSM_NB - 1) / TRSM_NB) * TRSM_NB * TRSM_NB;
+}
+
+template <typename T>
+void trsm_right_prepare(bool lower, bool trans, bool unit, long n, const T* a, long lda, T* inv, hipStream_t s) {
+  if (n <= 0) return;
+  const long nblk = (n + TRSM_NB - 1) / TRSM_NB;
+  trsm_inv_diag_kernel<T><<<(unsigned)nblk, 256, 0, s>>>(n, a, lda, trans, lower == trans, unit, inv, nullptr);
+  OBA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void trsm_right_prepared(bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b, long ldb,
+                         const T* inv, hipStream_t s) {
+  if (m <= 0 || n <= 0) return;
+  InvCtx<T> ic;
+  ic.a0 = a;
+  ic.lda = lda;
+  ic.inv = inv;
+  rec<T>(true, lower, trans, unit, m, n, a, lda, b, ldb, s, nullptr, ic);
+}
+
+template size_t trsm_right_inv_bytes<double>(long);
+template size_t trsm_right_inv_bytes<float>(long);
+template void trsm_right_prepare<double>(bool, bool, bool, long, const double*, long, double*, hipStream_t);
+template void trsm_right_prepare<float>(bool, bool, bool, long, const float*, long, float*, hipStream_t);
+template void trsm_right_prepared<double>(bool, bool, bool, long, long, const double*, long, double*, long,
+                                          const double*, hipStream_t);
+template void trsm_right_prepared<float>(bool, bool, bool, long, long, const float*, long, float*, long,
+                                         const float*, hipStream_t);
 template void trsm<double>(bool, bool, bool, bool, long, long, double, const double*, long, double*, long, hipStream_t, const int*);
 template void trsm<float>(bool, bool, bool, bool, long, long, float, const float*, long, float*, long, hipStream_t, const int*);
 

@@ -90,6 +90,51 @@ def test_getrf_rectangular(oba, m, n):
         assert np.abs(A - LUo).max() <= 200 * max(m, n) * EPS[dt] * max(1, np.abs(LUo).max())
 
 
+@pytest.mark.parametrize("m,n,nb", [(600, 600, 64), (700, 520, 64), (520, 700, 96), (1100, 1100, 256)])
+def test_getrf_lookahead_blocks(oba, monkeypatch, m, n, nb):
+    """The blocked look-ahead driver (trailing updates on a second stream,
+    block interchanges deferred to the columns outside the block), forced at
+    test sizes by a small block width: same pivots as the oracle, factors to
+    rounding, a zero column still reported at its position."""
+    L = oba.lib()
+    monkeypatch.setenv("OPENBLAS_AMD_LU_NB", str(nb))
+    rng = np.random.default_rng(m + 3 * n + nb)
+    A0 = np.asfortranarray(rng.standard_normal((m, n)))
+    A = A0.copy(order="F")
+    piv = np.zeros(min(m, n), np.int32)
+    info = c_int()
+    L.dgetrf_(oba.ip(m), oba.ip(n), A.ctypes.data, oba.ip(m), piv.ctypes.data, ctypes.byref(info))
+    LUo, pivo, infoo = oracle.getf2(A0)
+    assert info.value == infoo == 0
+    assert np.array_equal(piv, pivo)
+    assert np.abs(A - LUo).max() <= 100 * max(m, n) * EPS[np.float64] * max(1, np.abs(LUo).max())
+    # FP32: dget01 ratio and |l| <= 1 (pivots may differ from the oracle's on near-ties)
+    A32 = np.asfortranarray(A0.astype(np.float32))
+    p32 = np.zeros(min(m, n), np.int32)
+    L.sgetrf_(oba.ip(m), oba.ip(n), A32.ctypes.data, oba.ip(m), p32.ctypes.data, ctypes.byref(info))
+    assert info.value == 0
+    k = min(m, n)
+    Lm = (np.tril(A32, -1)[:, :k] + np.eye(m, k)).astype(np.float64)
+    Um = np.triu(A32)[:k, :].astype(np.float64)
+    assert np.abs(np.tril(A32, -1)).max() <= 1.0
+    PA = oracle.laswp(A0.astype(np.float32).astype(np.float64), 1, k, p32, 1)
+    assert np.linalg.norm(Lm @ Um - PA, 1) / (max(m, n) * np.linalg.norm(A0, 1) * EPS[np.float32]) < 30
+    if m == n:
+        # singular: exact zero column in the third block
+        Z0 = A0.copy(order="F"); Z0[:, 2 * nb + 5] = 0.0
+        Z = Z0.copy(order="F")
+        L.dgetrf_(oba.ip(n), oba.ip(n), Z.ctypes.data, oba.ip(n), piv.ctypes.data, ctypes.byref(info))
+        LUo, pivo, infoo = oracle.getf2(Z0)
+        assert info.value == infoo == 2 * nb + 6 and np.array_equal(piv, pivo)
+        # the solve through the tracked permutation
+        B0 = np.asfortranarray(rng.standard_normal((n, 3)))
+        A1, B = A0.copy(order="F"), B0.copy(order="F")
+        L.dgesv_(oba.ip(n), oba.ip(3), A1.ctypes.data, oba.ip(n), piv.ctypes.data, B.ctypes.data, oba.ip(n),
+                 ctypes.byref(info))
+        assert info.value == 0
+        assert np.abs(A0 @ B - B0).max() / (np.abs(A0).sum(1).max() * np.abs(B).max()) < 1e-14
+
+
 def test_getrf_singular_and_ties(oba):
     L = oba.lib()
     rng = np.random.default_rng(21)
