@@ -134,28 +134,74 @@ __device__ inline V bypass_load(const V* p) {
   return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// One column step of the panel with the column index J known at compile
-// time: row q's multiplier goes to element J, elements J+1..W-1 take the
-// rank-1 update with the pivot row u (zero beyond w, so padding columns stay
-// finite), and element J+1 becomes the row's candidate for the next column.
-// Every register index is static: the column loop dispatches here through a
-// uniform switch instead of indexing registers at run time.
+// x - l*u with one rounding (the update of every element of the panel)
+__device__ inline double lu_fms(double l, double u, double x) { return __builtin_fma(-l, u, x); }
+__device__ inline float lu_fms(float l, float u, float x) { return __builtin_fmaf(-l, u, x); }
+
+// Column step J of the panel with J known at compile time, in two parts.
+// Head: row q's multiplier goes to element J and element J+1 takes the
+// rank-1 update with the pivot row u, becoming the row's candidate for the
+// next column.  Tail: elements J+2..W-1 take the update (u is zero beyond w,
+// so padding columns stay finite).  Every register index is static: the
+// column loop dispatches here through a uniform switch instead of indexing
+// registers at run time.
 template <int J, typename T, int W, int RPT>
-__device__ __attribute__((always_inline)) inline void lu_col_step(T (&x)[RPT][W], const T (&lq)[RPT],
+__device__ __attribute__((always_inline)) inline void lu_col_head(T (&x)[RPT][W], const T (&lq)[RPT],
                                                                    const bool (&upd)[RPT], const T* u,
                                                                    T (&cand)[RPT]) {
 #pragma unroll
   for (int q = 0; q < RPT; q++) x[q][J] = upd[q] ? lq[q] : x[q][J];
+  if constexpr (J + 1 < W) {
+    const T uc = u[J + 1];
 #pragma unroll
-  for (int c = J + 1; c < W; c++) {
+    for (int q = 0; q < RPT; q++) {
+      x[q][J + 1] = upd[q] ? lu_fms(lq[q], uc, x[q][J + 1]) : x[q][J + 1];
+      cand[q] = x[q][J + 1];
+    }
+  }
+}
+
+template <int J, typename T, int W, int RPT>
+__device__ __attribute__((always_inline)) inline void lu_col_tail(T (&x)[RPT][W], const T (&lq)[RPT],
+                                                                   const bool (&upd)[RPT], const T* u) {
+#pragma unroll
+  for (int c = J + 2; c < W; c++) {
     const T uc = u[c];
 #pragma unroll
-    for (int q = 0; q < RPT; q++) x[q][c] = upd[q] ? x[q][c] - lq[q] * uc : x[q][c];
+    for (int q = 0; q < RPT; q++) x[q][c] = upd[q] ? lu_fms(lq[q], uc, x[q][c]) : x[q][c];
   }
-  if constexpr (J + 1 < W) {
+}
+
+// one step of a wavefront max-reduction of (key, position) through DPP lane
+// moves (CTRL: a DPP control word whose source lanes are all in the row)
+template <int CTRL>
+__device__ __attribute__((always_inline)) inline void lu_dpp_best(double& bv, int& bp) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(bv), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(bv), CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_update_dpp(0, bp, CTRL, 0xF, 0xF, false);
+  const double ov = __hiloint2double(hi, lo);
+  if (better(ov, op, bv, bp)) { bv = ov; bp = op; }
+}
+
+// the wavefront's best (key, position), returned in every lane: butterfly
+// within each row of 16 lanes (quad_perm xor 1, xor 2, half-row mirror, row
+// mirror), then the four row results combined as uniform values
+__device__ __attribute__((always_inline)) inline void lu_wave_best(double& bv, int& bp) {
+  lu_dpp_best<0xB1>(bv, bp);
+  lu_dpp_best<0x4E>(bv, bp);
+  lu_dpp_best<0x141>(bv, bp);
+  lu_dpp_best<0x140>(bv, bp);
+  const int blo = __double2loint(bv), bhi = __double2hiint(bv);
+  double v = __hiloint2double(__builtin_amdgcn_readlane(bhi, 0), __builtin_amdgcn_readlane(blo, 0));
+  int p = __builtin_amdgcn_readlane(bp, 0);
 #pragma unroll
-    for (int q = 0; q < RPT; q++) cand[q] = x[q][J + 1];
+  for (int r = 16; r < 64; r += 16) {
+    const double ov = __hiloint2double(__builtin_amdgcn_readlane(bhi, r), __builtin_amdgcn_readlane(blo, r));
+    const int op = __builtin_amdgcn_readlane(bp, r);
+    if (better(ov, op, v, p)) { v = ov; p = op; }
   }
+  bv = v;
+  bp = p;
 }
 
 template <int J, int W, typename F>
@@ -177,15 +223,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const long mp = m - j0;
   constexpr int DW = LuRec<T, W>::DW, RW = LuRec<T, W>::RW;
-  constexpr int RPI = 64 / DW;      // records per load instruction
+  constexpr int RPI = 64 / DW;         // records per load instruction
+  // LDS buffers that a wavefront may still read while a faster one starts
+  // the next column are double-buffered by column parity
   __shared__ double s_v[NWAVE];
   __shared__ int s_p[NWAVE];
-  __shared__ T s_row[W];            // the pivot row (P == 1)
-  __shared__ T s_cand[NWAVE][W];    // each wavefront's best row
-  __shared__ T s_rows[LU_PMAX][W];  // the P candidate rows of the column
-  __shared__ double s_wv[NWAVE];    // per wavefront: best record key,
-  __shared__ int s_wp[NWAVE];       // its position,
-  __shared__ int s_wr[NWAVE];       // its record (-2: timed out)
+  __shared__ T s_cand[2][NWAVE][W];    // each wavefront's best row
+  __shared__ T s_rows[2][LU_PMAX][W];  // the P candidate rows of the column (P > 1)
+  __shared__ double s_wv[NWAVE];       // per wavefront: best record key,
+  __shared__ int s_wp[NWAVE];          // its position,
+  __shared__ int s_wr[NWAVE];          // its record (-2: timed out)
 
   T x[RPT][W];
   T cand[RPT];      // element j of each row: the pivot candidates of column j
@@ -201,6 +248,15 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
     cand[q] = x[q][0];
   }
+  // With P > 1 the tail of column j's update (elements j+2..) is deferred
+  // into column j+1's exchange, where it overlaps the wait for the other
+  // workgroups' records; the published candidate rows get the same update
+  // (same operands, same rounding) on the way out.
+  T lqp[RPT];        // the previous column's multipliers,
+  bool updp[RPT];    // which rows took its update,
+  int frp = -1;      // and its pivot record in s_rows[(j-1)&1]
+#pragma unroll
+  for (int q = 0; q < RPT; q++) { lqp[q] = T(0); updp[q] = false; }
 
 #pragma unroll 1
   for (int j = 0; j < w; j++) {
@@ -215,20 +271,13 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
         if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; }
       }
     }
+    lu_wave_best(bv, bp);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(bv, off);
-      const int op = __shfl_xor(bp, off);
-      if (better(ov, op, bv, bp)) { bv = ov; bp = op; }
-    }
-    if (P > 1) {   // the wavefront's best row, for the publishing wavefront
+    for (int q = 0; q < RPT; q++)
+      if (live[q] && pos[q] == bp) {
 #pragma unroll
-      for (int q = 0; q < RPT; q++)
-        if (live[q] && pos[q] == bp) {
-#pragma unroll
-          for (int c = 0; c < W; c++) s_cand[wave][c] = x[q][c];
-        }
-    }
+        for (int c = 0; c < W; c++) s_cand[buf][wave][c] = x[q][c];
+      }
     if (lane == 0) { s_v[wave] = bv; s_p[wave] = bp; }
     __syncthreads();
     // (b) the workgroup's best (every thread reduces the same values)
@@ -239,29 +288,39 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; wk = k; }
     int gp;          // pivot position
     const T* urow;   // the pivot row in LDS
+    int fr = -1;
     if (P == 1) {
-#pragma unroll
-      for (int q = 0; q < RPT; q++)
-        if (live[q] && pos[q] == wp) {
-#pragma unroll
-          for (int c = 0; c < W; c++) s_row[c] = x[q][c];
-        }
-      __syncthreads();
       gp = wp;
-      urow = s_row;
+      urow = &s_cand[buf][wk][0];
     } else {
       const unsigned tag = base + (unsigned)(j + 1);
       const unsigned long long tg = (unsigned long long)tag << 32;
       unsigned long long* recs = rec + (size_t)buf * LU_PMAX * RW;
+      const T* uprev = &s_rows[buf ^ 1][frp < 0 ? 0 : frp][0];
       // (c) publish: word e of the record = {word e of the row, tag}; the
-      // last word {position (INT_MAX: no live row here), tag}
+      // last word {position (INT_MAX: no live row here), tag}.  The row gets
+      // the deferred tail of column j-1 first.
       if (wave == 0) {
         unsigned long long* dst = recs + (size_t)g * RW;
-        const unsigned* srow = reinterpret_cast<const unsigned*>(&s_cand[wk][0]);
-        if (lane < DW) wt_store(dst + lane, tg | srow[lane]);
+        if (lane < W) {
+          T v = s_cand[buf][wk][lane];
+          if (j > 0 && lane > j) v = lu_fms(s_cand[buf][wk][j - 1], uprev[lane], v);
+          if constexpr (sizeof(T) == 8) {
+            wt_store(dst + 2 * lane, tg | (unsigned)__double2loint(v));
+            wt_store(dst + 2 * lane + 1, tg | (unsigned)__double2hiint(v));
+          } else {
+            wt_store(dst + lane, tg | (unsigned)__float_as_int(v));
+          }
+        }
         if (lane == 0) wt_store(dst + DW, tg | (unsigned)wp);
       }
-      // (d) wavefront v collects records v*RPI.., (v+NWAVE)*RPI.., ...:
+      // (d) the deferred tail of column j-1 on this thread's rows
+      if (j > 0) {
+        lu_col_dispatch<0, W>(j - 1, [&](auto jc) __attribute__((always_inline)) {
+          lu_col_tail<decltype(jc)::value, T, W, RPT>(x, lqp, updp, uprev);
+        });
+      }
+      // (e) wavefront v collects records v*RPI.., (v+NWAVE)*RPI.., ...:
       // lane = h*DW + d loads word d of record r0+h, and word DW when d == 0
       const int h = lane / DW, d = lane % DW;
       double bv2 = -2.0;
@@ -284,7 +343,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
           if (++spins > (1L << 22)) { ok = false; break; }
         }
         if (!ok) break;
-        if (r < P) reinterpret_cast<unsigned*>(&s_rows[r][0])[d] = (unsigned)wd;
+        if (r < P) reinterpret_cast<unsigned*>(&s_rows[buf][r][0])[d] = (unsigned)wd;
         const int wdi = (int)(unsigned)wd, wqi = (int)(unsigned)wq;
 #pragma unroll
         for (int hh = 0; hh < RPI; hh++) {
@@ -307,9 +366,10 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       }
       if (lane == 0) { s_wv[wave] = bv2; s_wp[wave] = bp2; s_wr[wave] = ok ? br2 : -2; }
       __syncthreads();
-      // (e) the column's pivot: every thread reduces the same values
+      // (f) the column's pivot: every thread reduces the same values
       double fv = s_wv[0];
-      int fp = s_wp[0], fr = s_wr[0];
+      int fp = s_wp[0];
+      fr = s_wr[0];
       bool allok = fr != -2;
 #pragma unroll
       for (int k = 1; k < NWAVE; k++) {
@@ -322,7 +382,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
         break;
       }
       gp = fp;
-      urow = &s_rows[fr][0];
+      urow = &s_rows[buf][fr][0];
     }
     const T piv = urow[j];
     const bool nz = piv != T(0);
@@ -345,10 +405,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       lq[q] = doscale ? cand[q] * rinv : cand[q];
       upd[q] = true;
     }
+    const bool defer = P > 1;
     lu_col_dispatch<0, W>(j, [&](auto jc) __attribute__((always_inline)) {
-      lu_col_step<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow, cand);
+      lu_col_head<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow, cand);
+      if (!defer) lu_col_tail<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow);
     });
+#pragma unroll
+    for (int q = 0; q < RPT; q++) { lqp[q] = lq[q]; updp[q] = upd[q]; }
+    frp = fr;
   }
+  // the last deferred tail only touches elements >= w + 1: padding
   if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   // every row to its final position; list the rows that moved
 #pragma unroll
