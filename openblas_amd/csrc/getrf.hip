@@ -41,7 +41,8 @@ namespace oba {
 
 constexpr int LU_W = 32;        // leaf panel width
 constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
-constexpr int LU_TRSV_MAX_RHS = 4;   // solves with this many columns or fewer use trsv_lookback
+constexpr int LU_TRSV_MAX_RHS = 4;
+constexpr int LU_EB = 4;        // candidate record groups a wavefront polls at once   // solves with this many columns or fewer use trsv_lookback
 
 
 // ---- transposes (optionally demoting, with an overflow flag) --------------
@@ -110,7 +111,7 @@ template <typename T> struct LuNt { static constexpr int v = 512; };
 constexpr int LU_MV = 1 + 4 * LU_W;   // moves slot: count + 2*W (dst, src) pairs
 
 // rows per thread (registers: RPT * W values of T per thread)
-template <typename T> struct LuRpt { static constexpr int v = sizeof(T) == 8 ? 1 : 2; };
+template <typename T> struct LuRpt { static constexpr int v = 1; };
 
 // candidate records of one column: [2 buffers][LU_PMAX] records of
 // LuRec::RW words (double-buffered by column parity: a workgroup can only
@@ -137,40 +138,6 @@ __device__ inline V bypass_load(const V* p) {
 // x - l*u with one rounding (the update of every element of the panel)
 __device__ inline double lu_fms(double l, double u, double x) { return __builtin_fma(-l, u, x); }
 __device__ inline float lu_fms(float l, float u, float x) { return __builtin_fmaf(-l, u, x); }
-
-// Column step J of the panel with J known at compile time, in two parts.
-// Head: row q's multiplier goes to element J and element J+1 takes the
-// rank-1 update with the pivot row u, becoming the row's candidate for the
-// next column.  Tail: elements J+2..W-1 take the update (u is zero beyond w,
-// so padding columns stay finite).  Every register index is static: the
-// column loop dispatches here through a uniform switch instead of indexing
-// registers at run time.
-template <int J, typename T, int W, int RPT>
-__device__ __attribute__((always_inline)) inline void lu_col_head(T (&x)[RPT][W], const T (&lq)[RPT],
-                                                                   const bool (&upd)[RPT], const T* u,
-                                                                   T (&cand)[RPT]) {
-#pragma unroll
-  for (int q = 0; q < RPT; q++) x[q][J] = upd[q] ? lq[q] : x[q][J];
-  if constexpr (J + 1 < W) {
-    const T uc = u[J + 1];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-      x[q][J + 1] = upd[q] ? lu_fms(lq[q], uc, x[q][J + 1]) : x[q][J + 1];
-      cand[q] = x[q][J + 1];
-    }
-  }
-}
-
-template <int J, typename T, int W, int RPT>
-__device__ __attribute__((always_inline)) inline void lu_col_tail(T (&x)[RPT][W], const T (&lq)[RPT],
-                                                                   const bool (&upd)[RPT], const T* u) {
-#pragma unroll
-  for (int c = J + 2; c < W; c++) {
-    const T uc = u[c];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) x[q][c] = upd[q] ? lu_fms(lq[q], uc, x[q][c]) : x[q][c];
-  }
-}
 
 // one step of a wavefront max-reduction of (key, position) through DPP lane
 // moves (CTRL: a DPP control word whose source lanes are all in the row)
@@ -204,12 +171,41 @@ __device__ __attribute__((always_inline)) inline void lu_wave_best(double& bv, i
   bp = p;
 }
 
-template <int J, int W, typename F>
-__device__ __attribute__((always_inline)) inline void lu_col_dispatch(int j, F&& f) {
-  if constexpr (J < W) {
-    if (j == J) f(std::integral_constant<int, J>{});
-    else lu_col_dispatch<J + 1, W>(j, f);
+// x[k] -= l * u[k + OFF] for k in [K0, K1] with k <= kmax (kmax uniform;
+// u in LDS): whole chunks of 4 under one uniform branch, their 4 reads
+// issued together, selects only in the chunk that straddles kmax, no
+// per-element branch or wait
+template <int K0, int K1, int OFF, typename T, int W>
+__device__ __attribute__((always_inline)) inline void lu_tail_update(T (&x)[W], T l, const T* u, int kmax) {
+#pragma unroll
+  for (int c0 = K0; c0 <= K1; c0 += 4) {
+    if (c0 > kmax) break;
+    T uc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) uc[k] = c0 + k <= K1 ? u[c0 + k + OFF] : T(0);
+    if (c0 + 3 <= kmax) {
+#pragma unroll
+      for (int k = c0; k < c0 + 4; k++)
+        if (k <= K1) x[k] = lu_fms(l, uc[k - c0], x[k]);
+    } else {
+#pragma unroll
+      for (int k = c0; k < c0 + 4; k++) {
+        if (k <= K1) {
+          const T f = lu_fms(l, uc[k - c0], x[k]);
+          x[k] = k <= kmax ? f : x[k];
+        }
+      }
+    }
   }
+}
+
+// rotate a register row left by one element (static indices only)
+template <typename T, int W>
+__device__ __attribute__((always_inline)) inline void lu_rotate(T (&r)[W]) {
+  const T t0 = r[0];
+#pragma unroll
+  for (int k = 0; k < W - 1; k++) r[k] = r[k + 1];
+  r[W - 1] = t0;
 }
 
 template <typename T, int W, int RPT, int NT>
@@ -233,9 +229,15 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   __shared__ double s_wv[NWAVE];       // per wavefront: best record key,
   __shared__ int s_wp[NWAVE];          // its position,
   __shared__ int s_wr[NWAVE];          // its record (-2: timed out)
+  __shared__ int s_ipiv[W];            // the panel's pivots, written out at the end
+  int zinfo = 0;                       // first zero pivot (1-based, absolute)
 
+  // Row q of this thread, rotated: at column step j, x[q][k] holds element
+  // (j + k) mod W.  The current column is always x[q][0] and the column step
+  // is one code path with static register indices (no per-column code, which
+  // would not fit the instruction cache); rows in LDS and in the records are
+  // kept in the same rotated order.
   T x[RPT][W];
-  T cand[RPT];      // element j of each row: the pivot candidates of column j
   int pos[RPT];
   bool live[RPT];   // not yet a pivot row
 #pragma unroll
@@ -246,8 +248,11 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     const T* src = At + j0 + (j0 + r) * ldt;
 #pragma unroll
     for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
-    cand[q] = x[q][0];
   }
+  // the row loads complete here, once: otherwise the compiler keeps a
+  // vmcnt(0) wait inside the column loop, which then also waits for the
+  // write-through stores of the previous column
+  __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0) only (gfx9 encoding)
   // With P > 1 the tail of column j's update (elements j+2..) is deferred
   // into column j+1's exchange, where it overlaps the wait for the other
   // workgroups' records; the published candidate rows get the same update
@@ -261,13 +266,14 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
 #pragma unroll 1
   for (int j = 0; j < w; j++) {
     const int buf = j & 1;
+    const int kmax = W - 1 - j;   // x[.][1..kmax]: the columns right of j
     // (a) this thread's, then this wavefront's best: max key, lowest position
     double bv = -2.0;
     int bp = INT_MAX;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       if (live[q]) {
-        const double v = pivot_key(cand[q], pos[q], j);
+        const double v = pivot_key(x[q][0], pos[q], j);
         if (better(v, pos[q], bv, bp)) { bv = v; bp = pos[q]; }
       }
     }
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     for (int k = 1; k < NWAVE; k++)
       if (better(s_v[k], s_p[k], wv, wp)) { wv = s_v[k]; wp = s_p[k]; wk = k; }
     int gp;          // pivot position
-    const T* urow;   // the pivot row in LDS
+    const T* urow;   // the pivot row in LDS (rotated like x)
     int fr = -1;
     if (P == 1) {
       gp = wp;
@@ -296,15 +302,18 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       const unsigned tag = base + (unsigned)(j + 1);
       const unsigned long long tg = (unsigned long long)tag << 32;
       unsigned long long* recs = rec + (size_t)buf * LU_PMAX * RW;
+      // the previous pivot row, rotated for column j-1: its element k+1 is
+      // column j+k
       const T* uprev = &s_rows[buf ^ 1][frp < 0 ? 0 : frp][0];
       // (c) publish: word e of the record = {word e of the row, tag}; the
       // last word {position (INT_MAX: no live row here), tag}.  The row gets
-      // the deferred tail of column j-1 first.
+      // the deferred tail of column j-1 first (its multiplier of column j-1
+      // is element W-1).
       if (wave == 0) {
         unsigned long long* dst = recs + (size_t)g * RW;
         if (lane < W) {
           T v = s_cand[buf][wk][lane];
-          if (j > 0 && lane > j) v = lu_fms(s_cand[buf][wk][j - 1], uprev[lane], v);
+          if (j > 0 && lane >= 1 && lane <= kmax) v = lu_fms(s_cand[buf][wk][W - 1], uprev[lane + 1], v);
           if constexpr (sizeof(T) == 8) {
             wt_store(dst + 2 * lane, tg | (unsigned)__double2loint(v));
             wt_store(dst + 2 * lane + 1, tg | (unsigned)__double2hiint(v));
@@ -314,52 +323,76 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
         }
         if (lane == 0) wt_store(dst + DW, tg | (unsigned)wp);
       }
-      // (d) the deferred tail of column j-1 on this thread's rows
+      // (d) the deferred tail of column j-1 on this thread's rows (all
+      // pivot-row reads first, then selects: no per-element branch waits)
       if (j > 0) {
-        lu_col_dispatch<0, W>(j - 1, [&](auto jc) __attribute__((always_inline)) {
-          lu_col_tail<decltype(jc)::value, T, W, RPT>(x, lqp, updp, uprev);
-        });
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+          if (updp[q]) lu_tail_update<1, W - 2, 1, T, W>(x[q], lqp[q], uprev, kmax);
       }
       // (e) wavefront v collects records v*RPI.., (v+NWAVE)*RPI.., ...:
-      // lane = h*DW + d loads word d of record r0+h, and word DW when d == 0
+      // lane = h*DW + d loads word d of record r0+h, and word DW when d == 0.
+      // Up to LU_EB record groups are in flight at once (one round trip
+      // for all of them; polling them one after the other costs one round
+      // trip each).
+      constexpr int GS = NWAVE * RPI;   // record stride between a wavefront's groups
       const int h = lane / DW, d = lane % DW;
       double bv2 = -2.0;
       int bp2 = INT_MAX, br2 = -1;
       bool ok = true;
-      for (int r0 = wave * RPI; r0 < P; r0 += NWAVE * RPI) {
-        const int r = r0 + h;
-        const unsigned long long* src = recs + (size_t)r * RW;
-        unsigned long long wd = 0, wq = 0;
-        bool have = r >= P;
+      for (int c0 = wave * RPI; c0 < P; c0 += LU_EB * GS) {
+        unsigned long long wd[LU_EB], wq[LU_EB];
+        bool have[LU_EB];
+#pragma unroll
+        for (int k = 0; k < LU_EB; k++) {
+          have[k] = c0 + k * GS + h >= P;
+          wd[k] = 0;
+          wq[k] = 0;
+        }
         long spins = 0;
         while (true) {
-          if (!have) {
-            wd = bypass_load(src + d);
-            if (d == 0) wq = bypass_load(src + DW);
-            have = (unsigned)(wd >> 32) == tag && (d != 0 || (unsigned)(wq >> 32) == tag);
+#pragma unroll
+          for (int k = 0; k < LU_EB; k++) {
+            if (!have[k]) {
+              const unsigned long long* src = recs + (size_t)(c0 + k * GS + h) * RW;
+              wd[k] = bypass_load(src + d);
+              if (d == 0) wq[k] = bypass_load(src + DW);
+            }
           }
-          if (__all(have)) break;
+          bool all = true;
+#pragma unroll
+          for (int k = 0; k < LU_EB; k++) {
+            if (!have[k])
+              have[k] = (unsigned)(wd[k] >> 32) == tag && (d != 0 || (unsigned)(wq[k] >> 32) == tag);
+            all = all && have[k];
+          }
+          if (__all(all)) break;
           __builtin_amdgcn_s_sleep(1);
           if (++spins > (1L << 22)) { ok = false; break; }
         }
         if (!ok) break;
-        if (r < P) reinterpret_cast<unsigned*>(&s_rows[buf][r][0])[d] = (unsigned)wd;
-        const int wdi = (int)(unsigned)wd, wqi = (int)(unsigned)wq;
 #pragma unroll
-        for (int hh = 0; hh < RPI; hh++) {
-          if (r0 + hh < P) {
-            const int cp = __builtin_amdgcn_readlane(wqi, hh * DW);
-            T key;
-            if constexpr (sizeof(T) == 8) {
-              const int lo = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * j);
-              const int hi = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * j + 1);
-              key = __hiloint2double(hi, lo);
-            } else {
-              key = __int_as_float(__builtin_amdgcn_readlane(wdi, hh * DW + j));
-            }
-            if (cp != INT_MAX) {
-              const double vv = pivot_key(key, cp, j);
-              if (better(vv, cp, bv2, bp2)) { bv2 = vv; bp2 = cp; br2 = r0 + hh; }
+        for (int k = 0; k < LU_EB; k++) {
+          const int rb = c0 + k * GS;
+          if (rb >= P) break;
+          if (rb + h < P) reinterpret_cast<unsigned*>(&s_rows[buf][rb + h][0])[d] = (unsigned)wd[k];
+          const int wdi = (int)(unsigned)wd[k], wqi = (int)(unsigned)wq[k];
+#pragma unroll
+          for (int hh = 0; hh < RPI; hh++) {
+            if (rb + hh < P) {
+              const int cp = __builtin_amdgcn_readlane(wqi, hh * DW);
+              T key;   // element 0: column j
+              if constexpr (sizeof(T) == 8) {
+                const int lo = __builtin_amdgcn_readlane(wdi, hh * DW);
+                const int hi = __builtin_amdgcn_readlane(wdi, hh * DW + 1);
+                key = __hiloint2double(hi, lo);
+              } else {
+                key = __int_as_float(__builtin_amdgcn_readlane(wdi, hh * DW));
+              }
+              if (cp != INT_MAX) {
+                const double vv = pivot_key(key, cp, j);
+                if (better(vv, cp, bv2, bp2)) { bv2 = vv; bp2 = cp; br2 = rb + hh; }
+              }
             }
           }
         }
@@ -384,37 +417,54 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
       gp = fp;
       urow = &s_rows[buf][fr][0];
     }
-    const T piv = urow[j];
+    const T piv = urow[0];
     const bool nz = piv != T(0);
     const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
-    if (g == 0 && t == 0) {
-      ipiv[j0 + j] = (int)(j0 + gp + 1);
-      if (!nz && *info == 0) *info = (int)(j0 + j + 1);
-    }
+    if (t == 0) s_ipiv[j] = (int)(j0 + gp + 1);
+    if (!nz && zinfo == 0) zinfo = (int)(j0 + j + 1);
     const T rinv = doscale ? T(1) / piv : T(1);
-    // interchange (positions only) and multipliers; then the static step
+    // interchange (positions only), multipliers, the update of column j+1
+    // (and, with P == 1, of the rest), then the rotation to column j+1
+    const bool defer = P > 1;
     T lq[RPT];
     bool upd[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       upd[q] = false;
       lq[q] = T(0);
-      if (!live[q]) continue;
-      if (pos[q] == gp) { pos[q] = j; live[q] = false; continue; }   // the pivot row: now a row of U
-      if (pos[q] == j) pos[q] = gp;
-      lq[q] = doscale ? cand[q] * rinv : cand[q];
-      upd[q] = true;
+      if (live[q]) {
+        if (pos[q] == gp) {   // the pivot row: now a row of U
+          pos[q] = j;
+          live[q] = false;
+        } else {
+          if (pos[q] == j) pos[q] = gp;
+          lq[q] = doscale ? x[q][0] * rinv : x[q][0];
+          upd[q] = true;
+        }
+      }
+      if (upd[q]) {
+        x[q][0] = lq[q];
+        if (kmax >= 1) x[q][1] = lu_fms(lq[q], urow[1], x[q][1]);
+        if (!defer) lu_tail_update<2, W - 1, 0, T, W>(x[q], lq[q], urow, kmax);
+      }
+      lu_rotate<T, W>(x[q]);
     }
-    const bool defer = P > 1;
-    lu_col_dispatch<0, W>(j, [&](auto jc) __attribute__((always_inline)) {
-      lu_col_head<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow, cand);
-      if (!defer) lu_col_tail<decltype(jc)::value, T, W, RPT>(x, lq, upd, urow);
-    });
 #pragma unroll
     for (int q = 0; q < RPT; q++) { lqp[q] = lq[q]; updp[q] = upd[q]; }
     frp = fr;
   }
-  // the last deferred tail only touches elements >= w + 1: padding
+  if (g == 0) {   // the pivots and the first zero pivot
+    __syncthreads();
+    if (t < w) ipiv[j0 + t] = s_ipiv[t];
+    if (t == 0 && zinfo != 0 && *info == 0) *info = zinfo;
+  }
+  // the last deferred tail only touches elements >= w + 1 (padding); turn
+  // the rows back to column order
+#pragma unroll 1
+  for (int jj = w; jj < W; jj++) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) lu_rotate<T, W>(x[q]);
+  }
   if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   // every row to its final position; list the rows that moved
 #pragma unroll
