@@ -212,7 +212,7 @@ template <typename T, int W, int RPT, int NT>
 __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
                                                       int* __restrict__ ipiv, int* __restrict__ info,
                                                       int* __restrict__ moves, unsigned long long* __restrict__ rec,
-                                                      unsigned base, unsigned* err) {
+                                                      unsigned base, unsigned* err, int pre) {
   static_assert(W <= 64, "one pivot-row element per lane");
   constexpr int ROWS = NT * RPT;
   constexpr int NWAVE = NT / 64;
@@ -248,6 +248,49 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     const T* src = At + j0 + (j0 + r) * ldt;
 #pragma unroll
     for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
+  }
+  // pre: this panel is the right half of a 2W-wide node whose left half
+  // [j0-W, j0) was just factored and its interchanges applied here.  The
+  // node's TRSM and GEMM happen in this kernel, before the column loop:
+  // U12 = L11^{-1} A12 (every workgroup solves the W x w block in LDS; WG 0
+  // stores it at the end, after every workgroup has read A12, which the
+  // first column's exchange guarantees), then each row takes
+  // x -= l(row) * U12 with its W multipliers of the left half.
+  __shared__ T s_u12[W][W];
+  if (pre) {
+    T* stg = &s_rows[0][0][0];   // L11 | A12 staging, free before the loop
+    // At column j0-W+k, rows j0-W .. j0+W: row k of L11 (first W), of A12
+    for (int idx = t; idx < W * 2 * W; idx += NT) {
+      const int k = idx / (2 * W), e = idx % (2 * W);
+      stg[idx] = (e < W || e - W < w) ? At[(j0 - W + e) + (j0 - W + k) * ldt] : T(0);
+    }
+    __syncthreads();
+    if (t < W) {   // column t of U12: forward substitution with unit L11
+      T u[W];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        T v = stg[k * 2 * W + W + t];
+#pragma unroll
+        for (int kk = 0; kk < k; kk++) v = lu_fms(stg[k * 2 * W + kk], u[kk], v);
+        u[k] = v;
+      }
+#pragma unroll
+      for (int k = 0; k < W; k++) s_u12[k][t] = u[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      if (live[q]) {
+        const long r = (long)g * ROWS + q * NT + t;
+        const T* lr = At + (j0 - W) + (j0 + r) * ldt;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+          const T l = lr[k];
+#pragma unroll
+          for (int c = 0; c < W; c++) x[q][c] = lu_fms(l, s_u12[k][c], x[q][c]);
+        }
+      }
+    }
   }
   // the row loads complete here, once: otherwise the compiler keeps a
   // vmcnt(0) wait inside the column loop, which then also waits for the
@@ -453,10 +496,16 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
     for (int q = 0; q < RPT; q++) { lqp[q] = lq[q]; updp[q] = upd[q]; }
     frp = fr;
   }
-  if (g == 0) {   // the pivots and the first zero pivot
+  if (g == 0) {   // the pivots and the first zero pivot; U12
     __syncthreads();
     if (t < w) ipiv[j0 + t] = s_ipiv[t];
     if (t == 0 && zinfo != 0 && *info == 0) *info = zinfo;
+    if (pre) {
+      for (int idx = t; idx < W * w; idx += NT) {
+        const int k = idx / w, c = idx % w;
+        At[(j0 + c) + (j0 - W + k) * ldt] = s_u12[k][c];
+      }
+    }
   }
   // the last deferred tail only touches elements >= w + 1 (padding); turn
   // the rows back to column order
@@ -556,7 +605,7 @@ struct LuCtx {
 };
 
 template <typename T>
-static void lu_panel(LuCtx<T>& c, long j0, int w) {
+static void lu_panel(LuCtx<T>& c, long j0, int w, bool pre = false) {
   const long mp = c.m - j0;
   constexpr int RPT = LuRpt<T>::v;
   constexpr int NT = LuNt<T>::v;
@@ -565,7 +614,7 @@ static void lu_panel(LuCtx<T>& c, long j0, int w) {
   if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
   int* mv = c.moves + c.panel_no * LU_MV;
   lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
-                                                                        c.rec, c.base, c.err);
+                                                                        c.rec, c.base, c.err, pre ? 1 : 0);
   OBA_HIP(hipGetLastError());
   if (P > 1) c.base += (unsigned)w;   // record tags of this panel: base+1 .. base+w
   c.panel_no++;
@@ -576,6 +625,11 @@ static void lu_panel(LuCtx<T>& c, long j0, int w) {
 template <typename T>
 static void lu_rec(LuCtx<T>& c, long j0, long nc) {
   if (nc <= LU_W) { lu_panel<T>(c, j0, (int)nc); return; }
+  if (nc <= 2 * LU_W) {   // two leaves: the node's TRSM + GEMM run inside the right leaf's kernel
+    lu_panel<T>(c, j0, LU_W);
+    lu_panel<T>(c, j0 + LU_W, (int)(nc - LU_W), true);
+    return;
+  }
   long n1 = ((nc / 2) + LU_W - 1) / LU_W * LU_W;
   if (n1 >= nc) n1 = nc - LU_W;
   const long n2 = nc - n1;
