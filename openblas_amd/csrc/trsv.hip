@@ -8,13 +8,15 @@
 // dependency order (a workgroup only waits for blocks handed out before its
 // own, which are running or done, so the grid cannot deadlock).  A block's
 // workgroup first stages its diagonal block, then accumulates the
-// contributions of the solved blocks as their ready flags appear (each
+// contributions of the solved blocks as their solutions appear (each
 // wavefront a quarter of every tile, loads coalesced along whichever index
 // of A is contiguous), then wavefront 0 solves the diagonal block with
-// v_readlane broadcasts and publishes its part of x: write-through stores,
-// a wait for their completion, then a write-through flag; readers poll the
-// flag and read x with cache-bypassing loads, so no cache writeback or
-// invalidation (expensive on this part) happens per tile.
+// v_readlane broadcasts and publishes its part of x as tagged words: every
+// 32 bits of the solution paired with a nonzero tag in one 8-byte
+// write-through store (seen whole or not at all).  Readers poll those words
+// with cache-bypassing loads until every tag is set, so one round trip both
+// shows that a block is solved and delivers its values; no flag, no wait for
+// store completion, no cache writeback or invalidation.
 // The recursion-based TRSM needs n/64 dependent launches for the same job.
 #include "kernels.h"
 #include "lu_internal.h"
@@ -30,22 +32,20 @@ __device__ inline double tv_bcast(double v, int l) {
 }
 __device__ inline float tv_bcast(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
-template <typename T>
-__device__ inline T tv_wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
 // ROWLANES: rs == 1 (rows contiguous: lanes over rows); else cs == 1 (lanes
 // over columns).
+// published x of a block: TV_B values as 32-bit words, each {word, tag 1}
+template <typename T> struct TvPub { static constexpr int WPV = sizeof(T) / 4; };
+
 template <typename T, bool ROWLANES>
 __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restrict__ A, long rs, long cs, bool lower,
-                                                      bool unit, T* __restrict__ x, int* __restrict__ flags,
+                                                      bool unit, T* __restrict__ x, unsigned long long* __restrict__ pub,
                                                       unsigned* __restrict__ ticket, unsigned* __restrict__ err) {
+  constexpr int WPV = TvPub<T>::WPV;
   constexpr int LD = TV_B + 1;
   __shared__ T Md[TV_B * LD];      // M(r0+i, r0+k) at Md[i*LD + k]
   __shared__ T part[4][TV_B];
+  __shared__ T s_red[ROWLANES ? 1 : TV_B * LD];   // column partial sums, transposed
   __shared__ T dinv[TV_B];
   __shared__ int s_blk;
   const int t = threadIdx.x, lane = t & 63;
@@ -85,23 +85,39 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
         av[q] = (i < nr && lane < nk) ? A[(r0 + i) * rs + (k0 + lane) * cs] : T(0);
       }
     }
-    int ok = 1;
-    if (lane == 0) {
-      long spins = 0;
-      while (__hip_atomic_load(&flags[kb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0) {
-        __builtin_amdgcn_s_sleep(1);   // back off: every block's wavefronts poll
-        ++spins;
-        if (spins > (1L << 22) ||
-            ((spins & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-          atomicExch(err, 1u);
-          ok = 0;
-          break;
-        }
+    // block kb's solution: poll its tagged words (lane e: value e)
+    const unsigned long long* src = pub + (size_t)kb * TV_B * WPV;
+    unsigned long long wv[WPV];
+    bool have = lane >= nk;
+#pragma unroll
+    for (int h = 0; h < WPV; h++) wv[h] = 0;
+    long spins = 0;
+    bool ok = true;
+    while (true) {
+      if (!have) {
+#pragma unroll
+        for (int h = 0; h < WPV; h++) wv[h] = __hip_atomic_load(const_cast<unsigned long long*>(src + lane * WPV + h),
+                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        have = true;
+#pragma unroll
+        for (int h = 0; h < WPV; h++) have = have && (wv[h] >> 32) != 0;
+      }
+      if (__all(have)) break;
+      __builtin_amdgcn_s_sleep(1);   // back off: every block's wavefronts poll
+      ++spins;
+      if (spins > (1L << 22) ||
+          ((spins & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        if (lane == 0) atomicExch(err, 1u);
+        ok = false;
+        break;
       }
     }
-    if (!__builtin_amdgcn_readfirstlane(ok)) { bad = true; break; }
-    // issued after the flag was seen; reads memory, not a stale cache line
-    const T yl = lane < nk ? __hip_atomic_load(&x[k0 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : T(0);
+    if (!ok) { bad = true; break; }
+    T yl = T(0);
+    if (lane < nk) {
+      if constexpr (sizeof(T) == 8) yl = __hiloint2double((int)(unsigned)wv[1], (int)(unsigned)wv[0]);
+      else yl = __int_as_float((int)(unsigned)wv[0]);
+    }
     if (ROWLANES) {
 #pragma unroll
       for (int q = 0; q < 16; q++) acc += av[q] * tv_bcast(yl, w * 16 + q);   // av = 0 outside the tile
@@ -110,19 +126,25 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
       for (int q = 0; q < 16; q++) accv[q] += av[q] * yl;
     }
   }
+  // per-lane partial sums: by rows (ROWLANES) or, transposed through LDS,
+  // by columns, summed along each row by one lane
   if (ROWLANES) {
     part[w][lane] = acc;
   } else {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const T v = tv_wave_sum(accv[q]);
-      if (lane == 0) part[0][w * 16 + q] = v;
-    }
+    for (int q = 0; q < 16; q++) s_red[(w * 16 + q) * LD + lane] = accv[q];
   }
   __syncthreads();
   if (w != 0 || bad) return;
   T v = lane < nr ? x[r0 + lane] : T(0);
-  v -= ROWLANES ? (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]) : part[0][lane];
+  if (ROWLANES) {
+    v -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  } else {
+    T sacc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int l = 0; l < TV_B; l++) sacc[l & 3] += s_red[lane * LD + l];
+    v -= (sacc[0] + sacc[1]) + (sacc[2] + sacc[3]);
+  }
   // the diagonal block's row of this lane in registers; static steps
   T md[TV_B];
 #pragma unroll
@@ -147,11 +169,18 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
       }
     }
   }
-  // write-through stores of this block's solution, wait for their
-  // completion, then the (write-through) flag: no cache writeback needed
-  if (lane < nr) __hip_atomic_store(&x[r0 + lane], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  if (lane == 0) __hip_atomic_store(&flags[bi], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  // the tagged words for the later blocks, then the result itself
+  if (lane < nr) {
+    unsigned long long* dst = pub + (size_t)bi * TV_B * WPV + lane * WPV;
+    const unsigned long long tg = 1ull << 32;
+    if constexpr (sizeof(T) == 8) {
+      __hip_atomic_store(dst, tg | (unsigned)__double2loint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(dst + 1, tg | (unsigned)__double2hiint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+      __hip_atomic_store(dst, tg | (unsigned)__float_as_int(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    x[r0 + lane] = v;
+  }
 }
 
 template <typename T>
@@ -159,15 +188,16 @@ void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, 
   if (n <= 0) return;
   if (rs != 1 && cs != 1) fatal("trsv_lookback: one index of A must be contiguous", nullptr);
   const long nblk = (n + TV_B - 1) / TV_B;
-  DevBuf ws(sizeof(int) * (size_t)(nblk + 64));
-  int* flags = ws.as<int>();
-  unsigned* ticket = (unsigned*)(flags + nblk);
+  const size_t pub_bytes = sizeof(unsigned long long) * (size_t)nblk * TV_B * TvPub<T>::WPV;
+  DevBuf ws(256 + pub_bytes);
+  unsigned* ticket = ws.as<unsigned>();
   unsigned* err = ticket + 16;
-  OBA_HIP(hipMemsetAsync(ws.p, 0, sizeof(int) * (size_t)(nblk + 64), s));
+  unsigned long long* pub = (unsigned long long*)(ws.as<char>() + 256);
+  OBA_HIP(hipMemsetAsync(ws.p, 0, 256 + pub_bytes, s));
   if (rs == 1)
-    trsv_lb_kernel<T, true><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, flags, ticket, err);
+    trsv_lb_kernel<T, true><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
   else
-    trsv_lb_kernel<T, false><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, flags, ticket, err);
+    trsv_lb_kernel<T, false><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
   OBA_HIP(hipGetLastError());
   unsigned h = 0;
   OBA_HIP(hipMemcpyAsync(&h, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
