@@ -41,8 +41,8 @@ namespace oba {
 
 constexpr int LU_W = 32;        // leaf panel width
 constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
-constexpr int LU_TRSV_MAX_RHS = 4;
-constexpr int LU_EB = 4;        // candidate record groups a wavefront polls at once   // solves with this many columns or fewer use trsv_lookback
+constexpr int LU_TRSV_MAX_RHS = 4;   // solves with this many columns or fewer use trsv_lookback
+constexpr int LU_EB = 4;        // candidate record groups a wavefront polls at once
 
 
 // ---- transposes (optionally demoting, with an overflow flag) --------------
@@ -588,6 +588,53 @@ struct LuLeaf {
   int* mv;
 };
 
+// the leaves of one block, in order, for lu_swap_block_kernel
+constexpr int LU_MAXL = 16;
+struct LuLeafSet {
+  long j0[LU_MAXL];
+  const int* mv[LU_MAXL];
+  int n;
+};
+
+// The interchanges of all leaves of a block, in order, on the At rows in
+// [lo, hi) outside [ex_lo, ex_hi): lu_swap_kernel's gather/scatter once per
+// leaf inside one launch (each workgroup owns its 64-row slab throughout).
+template <typename T, int W>
+__global__ __launch_bounds__(256) void lu_swap_block_kernel(T* __restrict__ At, long ldt, long lo, long hi, long ex_lo,
+                                                            long ex_hi, long m, LuLeafSet ls) {
+  constexpr int RB = 64;
+  __shared__ int sd[2 * W], ss[2 * W];
+  __shared__ int s_cnt;
+  __shared__ T vals[2 * W][RB];
+  const long r0 = lo + (long)blockIdx.x * RB;
+  for (int l = 0; l < ls.n; l++) {
+    const long j0 = ls.j0[l], mp = m - j0;
+    const int* moves = ls.mv[l];
+    if (threadIdx.x == 0) s_cnt = min(max(moves[0], 0), 2 * W);
+    __syncthreads();
+    const int cnt = s_cnt;
+    if ((int)threadIdx.x < cnt) {
+      int d = moves[1 + 2 * threadIdx.x], sr = moves[2 + 2 * threadIdx.x];
+      if (d < 0 || d >= mp || sr < 0 || sr >= mp) d = sr = 0;   // never index outside the panel rows
+      sd[threadIdx.x] = d;
+      ss[threadIdx.x] = sr;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
+      const int k = idx / RB, rr = idx % RB;
+      const long r = r0 + rr;
+      if (r < hi && !(r >= ex_lo && r < ex_hi)) vals[k][rr] = At[r + (j0 + ss[k]) * ldt];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
+      const int k = idx / RB, rr = idx % RB;
+      const long r = r0 + rr;
+      if (r < hi && !(r >= ex_lo && r < ex_hi)) At[r + (j0 + sd[k]) * ldt] = vals[k][rr];
+    }
+    __syncthreads();   // before the next leaf's moves overwrite s_cnt, sd, ss and vals
+  }
+}
+
 template <typename T>
 struct LuCtx {
   T* At;
@@ -679,8 +726,14 @@ static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
       OBA_HIP(hipStreamWaitEvent(M, r.ev_b, 0));
       pending = false;
     }
-    for (size_t l = l0; l < c.leaves.size(); l++)
-      lu_swap<T>(At, ldt, 0, c.nrows_t, jb, jn, c.leaves[l].j0, c.m - c.leaves[l].j0, c.leaves[l].mv, M);
+    for (size_t l = l0; l < c.leaves.size(); l += LU_MAXL) {   // the block's interchanges outside it
+      LuLeafSet ls;
+      ls.n = (int)std::min<size_t>(LU_MAXL, c.leaves.size() - l);
+      for (int i = 0; i < ls.n; i++) { ls.j0[i] = c.leaves[l + i].j0; ls.mv[i] = c.leaves[l + i].mv; }
+      lu_swap_block_kernel<T, LU_W><<<(unsigned)((c.nrows_t + 63) / 64), 256, 0, M>>>(At, ldt, 0, c.nrows_t, jb, jn,
+                                                                                    c.m, ls);
+      OBA_HIP(hipGetLastError());
+    }
     if (jn >= mn) break;
     const long w1 = std::min(nb, mn - jn);
     const long below = c.m - jn;
@@ -710,8 +763,12 @@ static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
 
 // block width of the look-ahead driver (OPENBLAS_AMD_LU_NB; 0 turns it off);
 // it is used from 4 blocks on
+template <typename T>
 static long lu_block_nb() {
-  long v = 256;   // swept at n = 16384 over 128..1024 (with 16..64 CUs kept free)
+  // swept at n = 16384 over 128..1024 (with 16..128 CUs kept free): the
+  // FP32 factorization is bound by the panel chain, the FP64 one also by
+  // the trailing GEMM, which prefers the deeper updates of wider blocks
+  long v = sizeof(T) == 8 ? 512 : 256;
   if (const char* e = getenv("OPENBLAS_AMD_LU_NB")) v = atol(e);
   return v;
 }
@@ -736,7 +793,7 @@ int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hip
     iota_row_kernel<T><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(At + n, m, ldt);
     OBA_HIP(hipGetLastError());
   }
-  const long nb = lu_block_nb();
+  const long nb = lu_block_nb<T>();
   if (nb >= 2 * LU_W && nb % LU_W == 0 && mn >= 4 * nb) {
     lu_blocked<T>(c, mn, nb);
   } else {
