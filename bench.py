@@ -494,7 +494,7 @@ def main():
         "dgemm_8192_tflops": g_flops / 1e12,
         "dgemm_1024_tflops": g1_flops / 1e12,
         "sgemm_8192_tflops": s_flops / 1e12,
-        "roofline": {"kernel": "gemm_kernel<double,128,128,16> (dgemm 8192^3 NN)", "bound": "mfma",
+        "roofline": {"kernel": "gemm_kernel<double,128,128,4> (dgemm 8192^3 NN)", "bound": "mfma",
                      "achieved": g_flops / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": g_flops / PEAK_FP64, "traffic": profiled_traffic("dgemm_8192_nn_fp64.md"),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, calibrated)"},
