@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BDesc* __restrict__ de
 template <typename T, int RPT, int W>
 __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ desc, int count,
                                                      unsigned* __restrict__ counter, int* __restrict__ info_out) {
-  constexpr int NT = 256, MAXM = NT * RPT, UB = NT, LDU = W + 1, CH = 8;
+  constexpr int NT = 256, MAXM = NT * RPT, UB = NT, LDU = W + 1, CH = 8, BU = 8;
   typedef typename RowVec<T, W>::type RV;
   __shared__ int s_perm[MAXM];            // position -> physical row
   __shared__ T s_row[2][4][W];            // per-wave pivot candidates (by column parity)
@@ -365,14 +365,24 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
         for (int q = 0; q < RPT; q++) {
           if (!live[q] || pos[q] < j0 + w) continue;
           T* rowp = A + phys[q];
+          // BU columns per step, their loads issued together (one exposed
+          // memory latency per step instead of per column)
           #pragma unroll 1
-          for (int cc = 0; cc < cw; cc++) {
-            T a = rowp[(c0 + cc) * lda];
-            const T* uc = s_u + cc * LDU;
+          for (int cb = 0; cb < cw; cb += BU) {
+            T a[BU];
 #pragma unroll
-            for (int k = 0; k < W; k++)
-              if (k < w) a -= x[q][k] * uc[k];
-            rowp[(c0 + cc) * lda] = a;
+            for (int u = 0; u < BU; u++) a[u] = cb + u < cw ? rowp[(long)(c0 + cb + u) * lda] : T(0);
+#pragma unroll
+            for (int u = 0; u < BU; u++) {
+              if (cb + u < cw) {
+                const T* uc = s_u + (cb + u) * LDU;
+                T v = a[u];
+#pragma unroll
+                for (int k = 0; k < W; k++)
+                  if (k < w) v -= x[q][k] * uc[k];
+                rowp[(long)(c0 + cb + u) * lda] = v;
+              }
+            }
           }
         }
         __syncthreads();
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
 template <typename T, int RPT, int W, bool LOWER>
 __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ desc, int count,
                                                      unsigned* __restrict__ counter, int* __restrict__ info_out) {
-  constexpr int NT = 256, UB = NT, LDL = W + 1;
+  constexpr int NT = 256, UB = NT, LDL = W + 1, BU = 8;
   typedef typename RowVec<T, W>::type RV;
   __shared__ T s_col[2][W];
   __shared__ T s_l[UB * LDL];
@@ -511,14 +521,23 @@ __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ d
         for (int q = 0; q < RPT; q++) {
           if (!live[q] || row[q] < c0) continue;
           const int cend = min(row[q], c0 + UB - 1);
+          // BU columns per step, their loads issued together
           #pragma unroll 1
-          for (int c = c0; c <= cend; c++) {
-            T a = at(row[q], c);
-            const T* lr = s_l + (c - c0) * LDL;
+          for (int cb = c0; cb <= cend; cb += BU) {
+            T a[BU];
 #pragma unroll
-            for (int k = 0; k < W; k++)
-              if (k < w) a -= x[q][k] * lr[k];
-            at(row[q], c) = a;
+            for (int u = 0; u < BU; u++) a[u] = cb + u <= cend ? at(row[q], cb + u) : T(0);
+#pragma unroll
+            for (int u = 0; u < BU; u++) {
+              if (cb + u <= cend) {
+                const T* lr = s_l + (cb + u - c0) * LDL;
+                T v = a[u];
+#pragma unroll
+                for (int k = 0; k < W; k++)
+                  if (k < w) v -= x[q][k] * lr[k];
+                at(row[q], cb + u) = v;
+              }
+            }
           }
         }
         __syncthreads();
