@@ -21,6 +21,7 @@
 // accumulator stay in registers, and C is read and written once.
 #include "kernels.h"
 #include "ozaki.h"
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -32,14 +33,32 @@ constexpr int OZ_BK = 64;       // k per slab (one i8 MFMA k-depth)
 constexpr int OZ_T = 64;        // C tile edge
 
 // element (v, k) of an operand "vector" v at X[v*sv + k*sk]
-__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* vmax, double* vsum) {
-  const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;   // sv == 1
+// sv == 1: the k range is cut into gridDim.y chunks (one vector per thread
+// and chunk, loads coalesced across the vectors); partial max/sum per chunk,
+// combined in chunk order by oz_stats_combine (deterministic)
+__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* pmax, double* psum) {
+  const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (v >= V) return;
+  const long kc = (K + gridDim.y - 1) / gridDim.y, k0 = blockIdx.y * kc, k1 = min(K, k0 + kc);
   double mx = 0.0, sm = 0.0;
-  for (long k = 0; k < K; k++) {
+  for (long k = k0; k < k1; k++) {
     double a = fabs(X[v + k * sk]);
     mx = a > mx ? a : mx;
     sm += a;
+  }
+  pmax[blockIdx.y * V + v] = mx;
+  psum[blockIdx.y * V + v] = sm;
+}
+
+__global__ void oz_stats_combine(long V, int parts, const double* __restrict__ pmax, const double* __restrict__ psum,
+                                 double* vmax, double* vsum) {
+  const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double mx = 0.0, sm = 0.0;
+  for (int p = 0; p < parts; p++) {
+    const double a = pmax[p * V + v];
+    mx = a > mx ? a : mx;
+    sm += psum[p * V + v];
   }
   vmax[v] = mx;
   vsum[v] = sm;
@@ -210,7 +229,15 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
 static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vsum, hipStream_t s) {
   if (V <= 0) return;
   if (sv == 1) {
-    oz_stats_vcontig<<<(unsigned)((V + 255) / 256), 256, 0, s>>>(V, K, X, sk, vmax, vsum);
+    // enough workgroups to fill the device: ~1024 (vector blocks x k chunks)
+    const long vb = (V + 255) / 256;
+    const int parts = (int)std::max(1L, std::min((K + 63) / 64, 1024 / vb));
+    DevBuf part(sizeof(double) * 2 * (size_t)parts * V);
+    double* pmax = part.as<double>();
+    double* psum = pmax + (size_t)parts * V;
+    oz_stats_vcontig<<<dim3((unsigned)vb, (unsigned)parts), 256, 0, s>>>(V, K, X, sk, pmax, psum);
+    OBA_HIP(hipGetLastError());
+    oz_stats_combine<<<(unsigned)vb, 256, 0, s>>>(V, parts, pmax, psum, vmax, vsum);
   } else {
     oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vsum);
   }
