@@ -139,8 +139,18 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
   constexpr int RS = 80;   // padded LDS row stride in bytes
   __shared__ __attribute__((aligned(16))) signed char la[NS][OZ_T * RS];
   __shared__ __attribute__((aligned(16))) signed char lb[NS][OZ_T * RS];
-  const long tiles_m = (M + OZ_T - 1) / OZ_T;
-  const long tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+  // XCD-aware grouped tile order (workgroup b runs on XCD b%8): each XCD
+  // takes a contiguous range of an order that walks 8 tile rows at a time,
+  // so the slices of an A row block and a B column block are reused from
+  // that XCD's L2 instead of being streamed from HBM once per tile
+  const long tiles_m = (M + OZ_T - 1) / OZ_T, tiles_n = (N + OZ_T - 1) / OZ_T;
+  const long T_all = tiles_m * tiles_n;
+  long lin = blockIdx.x;
+  if ((T_all & 7) == 0) lin = (long)(blockIdx.x & 7) * (T_all >> 3) + (blockIdx.x >> 3);
+  constexpr long GM = 8;
+  const long group = lin / (GM * tiles_n), first_m = group * GM;
+  const long gsize = min(tiles_m - first_m, GM), in_group = lin - group * GM * tiles_n;
+  const long tm = first_m + in_group % gsize, tn = in_group / gsize;
   const long m0 = tm * OZ_T, n0 = tn * OZ_T;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
