@@ -172,14 +172,33 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[L][a][b] = i4v{0, 0, 0, 0};
+    // software pipeline: the next slab's global loads are in flight in
+    // registers while the current slab is multiplied out of LDS
+    // thread's share of a slab: row tid>>2, bytes (tid&3)*16.. of every slice
+    const int frow = threadIdx.x >> 2, fq = (threadIdx.x & 3) * 16;
+    const signed char* ga = SA + ((size_t)m0 + frow) * Kp + fq;
+    const signed char* gb = SB + ((size_t)n0 + frow) * Kp + fq;
+    i4v pa[NS], pb[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      pa[t] = *(const i4v*)(ga + (size_t)t * Mp * Kp + kc0);
+      pb[t] = *(const i4v*)(gb + (size_t)t * Np * Kp + kc0);
+    }
     for (long k0 = kc0; k0 < kc1; k0 += OZ_BK) {
       // stage the slab: NS slices x 64 rows x 64 bytes for each operand
-      for (int idx = threadIdx.x; idx < NS * OZ_T * 4; idx += 256) {
-        const int t = idx / (OZ_T * 4), rem = idx % (OZ_T * 4), row = rem >> 2, q = (rem & 3) * 16;
-        *(int4*)(&la[t][row * RS + q]) = *(const int4*)(SA + ((size_t)t * Mp + m0 + row) * Kp + k0 + q);
-        *(int4*)(&lb[t][row * RS + q]) = *(const int4*)(SB + ((size_t)t * Np + n0 + row) * Kp + k0 + q);
+#pragma unroll
+      for (int t = 0; t < NS; t++) {
+        *(i4v*)(&la[t][frow * RS + fq]) = pa[t];
+        *(i4v*)(&lb[t][frow * RS + fq]) = pb[t];
       }
       __syncthreads();
+      if (k0 + OZ_BK < kc1) {
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          pa[t] = *(const i4v*)(ga + (size_t)t * Mp * Kp + k0 + OZ_BK);
+          pb[t] = *(const i4v*)(gb + (size_t)t * Np * Kp + k0 + OZ_BK);
+        }
+      }
       // fragments: lane l holds row (l%16) of a 16-row block, k bytes (l/16)*16..+15
       i4v fb[NS][2];
 #pragma unroll
