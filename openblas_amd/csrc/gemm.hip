@@ -272,6 +272,88 @@ void scale(long m, long n, T beta, T* c, long ldc, hipStream_t s) {
   OBA_HIP(hipGetLastError());
 }
 
+// C = alpha*A*B + beta*C with few columns (N <= GS_NMAX, no transposes):
+// an HBM-bound matrix-vector product, where 64x64 MFMA tiles leave most of
+// the device idle (one tile column: 1.36 ms for 16384^2 x 1 FP64, 6x the
+// time to stream A).  Thread = row of A (coalesced down the columns), the k
+// range cut into gridDim.y chunks for enough workgroups; the chunk's B
+// values staged in LDS; per-chunk partial sums combined in chunk order by a
+// second kernel (deterministic).
+constexpr int GS_NMAX = 4;
+constexpr int GS_KC = 256;   // B rows staged per LDS round
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void gemm_skinny_part_kernel(long M, long K, long kchunk, const T* __restrict__ A,
+                                                               long lda, const T* __restrict__ B, long ldb,
+                                                               T* __restrict__ part) {
+  __shared__ T sb[NR][GS_KC];
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  const long k0 = (long)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+  T acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = T(0);
+  for (long kb = k0; kb < k1; kb += GS_KC) {
+    const int kn = (int)min((long)GS_KC, k1 - kb);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < NR * GS_KC; idx += 256) {
+      const int r = idx / GS_KC, kk = idx % GS_KC;
+      sb[r][kk] = kk < kn ? B[kb + kk + r * ldb] : T(0);
+    }
+    __syncthreads();
+    if (i < M) {
+      const T* ap = A + i + kb * lda;
+      int kk = 0;
+      for (; kk + 8 <= kn; kk += 8) {   // 8 loads in flight
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = ap[(long)(kk + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) acc[r] += a[u] * sb[r][kk + u];
+      }
+      for (; kk < kn; kk++) {
+        const T a = ap[(long)kk * lda];
+#pragma unroll
+        for (int r = 0; r < NR; r++) acc[r] += a * sb[r][kk];
+      }
+    }
+  }
+  if (i < M) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) part[((long)blockIdx.y * NR + r) * M + i] = acc[r];
+  }
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void gemm_skinny_sum_kernel(long M, int parts, const T* __restrict__ part, T alpha,
+                                                              T beta, T* __restrict__ C, long ldc) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= M) return;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    T s = T(0);
+    for (int p = 0; p < parts; p++) s += part[((long)p * NR + r) * M + i];
+    T* cp = C + i + r * ldc;
+    *cp = beta == T(0) ? alpha * s : alpha * s + beta * *cp;   // beta == 0: C is not read
+  }
+}
+
+template <typename T, int NR>
+static void gemm_skinny(long m, long k, T alpha, const T* a, long lda, const T* b, long ldb, T beta, T* c, long ldc,
+                        hipStream_t s) {
+  const long mb = (m + 255) / 256;
+  const int parts = (int)std::max(1L, std::min((k + GS_KC - 1) / GS_KC, (4L * rt().num_cu + mb - 1) / mb));
+  const long kchunk = ((k + parts - 1) / parts + GS_KC - 1) / GS_KC * GS_KC;
+  const int np = (int)((k + kchunk - 1) / kchunk);
+  DevBuf part(sizeof(T) * (size_t)np * NR * m);
+  gemm_skinny_part_kernel<T, NR><<<dim3((unsigned)mb, (unsigned)np), 256, 0, s>>>(m, k, kchunk, a, lda, b, ldb,
+                                                                                part.as<T>());
+  OBA_HIP(hipGetLastError());
+  gemm_skinny_sum_kernel<T, NR><<<(unsigned)mb, 256, 0, s>>>(m, np, part.as<T>(), alpha, beta, c, ldc);
+  OBA_HIP(hipGetLastError());
+}
+
 template <typename T, int BM, int BN, int BK, bool VEC>
 static void launch_cfg(bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
                        const T* B, long ldb, T beta, T* C, long ldc, hipStream_t s, int tri,
@@ -297,6 +379,14 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
     if (tri) fatal("gemm: triangular update with k==0 or alpha==0 is not supported", nullptr);
     scale<T>(m, n, beta, c, ldc, s);
     return;
+  }
+  if (!ta && !tb && n <= GS_NMAX && tri == 0 && abort_flag == nullptr && m >= 2048 && k >= 2048) {
+    switch (n) {
+      case 1: gemm_skinny<T, 1>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
+      case 2: gemm_skinny<T, 2>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
+      case 3: gemm_skinny<T, 3>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
+      default: gemm_skinny<T, 4>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
+    }
   }
   constexpr int V = Vec16<T>::n;
   const bool vec = (lda % V == 0) && (ldb % V == 0) && (((uintptr_t)a & 15) == 0) && (((uintptr_t)b & 15) == 0);

@@ -121,6 +121,28 @@ def test_gemm_odd_ld_and_edges(oba, dt):
     assert np.array_equal(C, np.zeros_like(C))
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_gemm_few_columns(oba, dt):
+    """The few-column path (N <= 4, no transposes, large M and K: split-k
+    matrix-vector products with an ordered combine): dmmch ratio for every
+    N in 1..4, odd leading dimensions, beta == 0 over NaN."""
+    L = oba.lib()
+    f = L.cblas_dgemm if dt == np.float64 else L.cblas_sgemm
+    rng = np.random.default_rng(17)
+    m, k, pad = 3001, 2600, 3
+    A = np.asfortranarray(rng.uniform(-1, 1, (m + pad, k)).astype(dt))
+    for n in (1, 2, 3, 4):
+        B = np.asfortranarray(rng.uniform(-1, 1, (k + 1, n)).astype(dt))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)).astype(dt))
+        C0 = C.copy(order="F")
+        f(102, 111, 111, m, n, k, -0.8, A.ctypes.data, m + pad, B.ctypes.data, k + 1, 1.1, C.ctypes.data, m + pad)
+        assert gemm_ratio(0, 0, -0.8, A[:m], B[:k], 1.1, C0[:m], C[:m], dt) < 16, n
+        C[:] = np.nan
+        f(102, 111, 111, m, n, k, 1.0, A.ctypes.data, m + pad, B.ctypes.data, k + 1, 0.0, C.ctypes.data, m + pad)
+        assert not np.isnan(C[:m]).any()
+        assert gemm_ratio(0, 0, 1.0, A[:m], B[:k], 0.0, np.zeros((m, n), dt), C[:m], dt) < 16, n
+
+
 def test_fortran_gemm_matches_cblas(oba):
     L = oba.lib()
     rng = np.random.default_rng(12)
