@@ -43,6 +43,13 @@
 
 namespace oba {
 
+// The problems' pointers come from the descriptor table, so the compiler
+// sees generic pointers and emits flat loads and stores, which also count
+// against the LDS wait counter: every LDS wait then waits for outstanding
+// memory operations too.  They point to device memory: use them as global.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
 // ---- small device helpers ----------------------------------------------------
 __device__ inline double bcast_lane(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -77,8 +84,8 @@ __global__ void bcopy_kernel(const BDesc* __restrict__ src, const BDesc* __restr
   const int* fs = &s.m;
   const long rows = fs[rows_field], cols = fs[cols_field];
   const long lds = which == 2 ? s.ldc : which ? s.ldb : s.lda, ldd = which == 2 ? d.ldc : which ? d.ldb : d.lda;
-  const T* a = (const T*)(which == 2 ? s.c : which ? s.b : s.a);
-  T* b = (T*)(which == 2 ? d.c : which ? d.b : d.a);
+  gptr<const T> a = (gptr<const T>)(which == 2 ? s.c : which ? s.b : s.a);
+  gptr<T> b = (gptr<T>)(which == 2 ? d.c : which ? d.b : d.a);
   const long total = rows * cols;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const long i = e % rows, j = e / rows;
@@ -129,9 +136,9 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BDesc* __restrict__ de
     const int2 wk = work[it];
     const BDesc& d = desc[wk.x];
     const long M = d.m, N = d.n, K = d.k, lda = d.lda, ldb = d.ldb, ldc = d.ldc;
-    const T* A = (const T*)d.a;
-    const T* B = (const T*)d.b;
-    T* C = (T*)d.c;
+    gptr<const T> A = (gptr<const T>)d.a;
+    gptr<const T> B = (gptr<const T>)d.b;
+    gptr<T> C = (gptr<T>)d.c;
     const T alpha = (T)d.alpha, beta = (T)d.beta;
     const int st_m = (int)((M + 63) / 64);
     const long sm0 = (long)(wk.y % st_m) * 64, sn0 = (long)(wk.y / st_m) * 64;
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BDesc* __restrict__ de
           for (int r = 0; r < 4; r++) {
             const long n = n0 + wn0 + j * 16 + mf::orow(lane, r);
             if (m < M && n < N) {
-              T* p = C + m + n * ldc;
+              auto p = C + m + n * ldc;
               T v = alpha * acc[i][j][r];
               if (!bz) v += beta * *p;
               *p = v;
@@ -243,8 +250,8 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
     const BDesc& d = desc[it];
     const int m = d.m, n = d.n, kmax = min(d.m, d.n);
     const long lda = d.lda;
-    T* A = (T*)d.a;
-    int* ipiv = d.ipiv;
+    gptr<T> A = (gptr<T>)d.a;
+    gptr<int> ipiv = (gptr<int>)d.ipiv;
     int info = 0;
     for (int i = t; i < m; i += NT) s_perm[i] = i;
     __syncthreads();
@@ -343,7 +350,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
       for (int c0 = j0 + w; c0 < n; c0 += UB) {
         const int cw = min(UB, n - c0);
         if (t < cw) {   // U12 column c0+t: forward substitution with unit L11
-          T* colp = A + (long)(c0 + t) * lda;
+          auto colp = A + (long)(c0 + t) * lda;
           T u[W];
 #pragma unroll
           for (int k = 0; k < W; k++) u[k] = (k < w) ? colp[s_perm[j0 + k]] : T(0);
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
           if (!live[q] || pos[q] < j0 + w) continue;
-          T* rowp = A + phys[q];
+          auto rowp = A + phys[q];
           // BU columns per step, their loads issued together (one exposed
           // memory latency per step instead of per column)
           #pragma unroll 1
@@ -447,8 +454,8 @@ __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ d
     const BDesc& d = desc[it];
     const int n = d.n;
     const long lda = d.lda;
-    T* A = (T*)d.a;
-    auto at = [&](long i, long c) -> T& { return LOWER ? A[i + c * lda] : A[c + i * lda]; };
+    gptr<T> A = (gptr<T>)d.a;
+    auto at = [&](long i, long c) -> decltype(A[0])& { return LOWER ? A[i + c * lda] : A[c + i * lda]; };
     int info = 0;
     #pragma unroll 1
     for (int j0 = 0; j0 < n; j0 += W) {
@@ -597,17 +604,17 @@ __global__ __launch_bounds__(256) void bgetrs_kernel(const BDesc* __restrict__ d
     const BDesc& d = desc[it];
     const int n = d.n, nrhs = d.k;
     const long lda = d.lda, ldb = d.ldb;
-    const T* A = (const T*)d.a;
-    T* B = (T*)d.b;
+    gptr<const T> A = (gptr<const T>)d.a;
+    gptr<T> B = (gptr<T>)d.b;
     int ip[S];
 #pragma unroll
     for (int s = 0; s < S; s++) {
       const int i = 64 * s + lane;
-      ip[s] = i < n ? d.ipiv[i] - 1 : i;
+      ip[s] = i < n ? ((gptr<const int>)d.ipiv)[i] - 1 : i;
     }
 #pragma unroll 1
     for (int r = 0; r < nrhs; r++) {
-      T* b = B + r * ldb;
+      auto b = B + r * ldb;
       T x[S];
 #pragma unroll
       for (int s = 0; s < S; s++) {
