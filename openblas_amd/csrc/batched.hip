@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BDesc* __restrict__ de
 template <typename T, int RPT, int W>
 __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ desc, int count,
                                                      unsigned* __restrict__ counter, int* __restrict__ info_out) {
-  constexpr int NT = 256, MAXM = NT * RPT, UB = NT, LDU = W + 1, CH = 8, BU = 8;
+  constexpr int NT = 256, MAXM = NT * RPT, UB = NT, LDU = W + 1, CH = 8, BU = RPT == 1 ? 32 : 16;
   typedef typename RowVec<T, W>::type RV;
   __shared__ int s_perm[MAXM];            // position -> physical row
   __shared__ T s_row[2][4][W];            // per-wave pivot candidates (by column parity)
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
 template <typename T, int RPT, int W, bool LOWER>
 __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ desc, int count,
                                                      unsigned* __restrict__ counter, int* __restrict__ info_out) {
-  constexpr int NT = 256, UB = NT, LDL = W + 1, BU = 8;
+  constexpr int NT = 256, UB = NT, LDL = W + 1, BU = RPT == 1 ? 32 : 16;
   typedef typename RowVec<T, W>::type RV;
   __shared__ T s_col[2][W];
   __shared__ T s_l[UB * LDL];
@@ -514,6 +514,62 @@ __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ d
           if (c < kdone && j0 + c <= row[q]) at(row[q], j0 + c) = x[q][c];
       }
       if (fail) { info = fail; break; }
+      if constexpr (RPT == 1) {
+        // trailing SYRK A22 -= L21 L21^T on MFMA 16x16 tiles of the lower
+        // triangle (4 wavefronts, GT tiles each per step, their C elements
+        // loaded before the products so one memory latency covers GT tiles)
+        typedef BMF<T> mf;
+        const int c0 = j0 + w, nr = n - c0;
+        if (nr > 0) {
+          if (live[0] && row[0] >= c0) {
+#pragma unroll
+            for (int c = 0; c < W; c++) s_l[(row[0] - c0) * LDL + c] = c < w ? x[0][c] : T(0);
+          }
+          __syncthreads();
+          const int lane = t & 63, wave = t >> 6;
+          const int nt = (nr + 15) / 16, ntiles = nt * (nt + 1) / 2;
+          constexpr int GT = 4;
+          #pragma unroll 1
+          for (int tb = wave * GT; tb < ntiles; tb += 4 * GT) {
+            int ti[GT], tj[GT];
+            T cv[GT][4];
+            typename mf::acc acc[GT];
+#pragma unroll
+            for (int g = 0; g < GT; g++) {
+              const int tile = min(tb + g, ntiles - 1);
+              int a0 = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+              while (a0 * (a0 + 1) / 2 > tile) a0--;
+              while ((a0 + 1) * (a0 + 2) / 2 <= tile) a0++;
+              ti[g] = a0;
+              tj[g] = tile - a0 * (a0 + 1) / 2;
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int ri = c0 + ti[g] * 16 + (lane & 15), cj = c0 + tj[g] * 16 + mf::orow(lane, r);
+                cv[g][r] = (tb + g < ntiles && ri < n && ri >= cj) ? at(ri, cj) : T(0);
+              }
+            }
+#pragma unroll
+            for (int g = 0; g < GT; g++) {
+              acc[g] = typename mf::acc{T(0), T(0), T(0), T(0)};
+              const int ra = ti[g] * 16 + (lane & 15), rb = tj[g] * 16 + (lane & 15);
+#pragma unroll
+              for (int kk = 0; kk < W; kk += 4) {
+                const int k = kk + (lane >> 4);
+                const T fa = ra < nr ? s_l[ra * LDL + k] : T(0);
+                const T fb = rb < nr ? s_l[rb * LDL + k] : T(0);
+                acc[g] = mf::mma(fb, fa, acc[g]);
+              }
+            }
+#pragma unroll
+            for (int g = 0; g < GT; g++)
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int ri = c0 + ti[g] * 16 + (lane & 15), cj = c0 + tj[g] * 16 + mf::orow(lane, r);
+                if (tb + g < ntiles && ri < n && ri >= cj) at(ri, cj) = cv[g][r] - acc[g][r];
+              }
+          }
+        }
+      } else {
       #pragma unroll 1
       for (int c0 = j0 + w; c0 < n; c0 += UB) {
 #pragma unroll
@@ -548,6 +604,7 @@ __global__ __launch_bounds__(256) void bpotrf_kernel(const BDesc* __restrict__ d
           }
         }
         __syncthreads();
+      }
       }
       __syncthreads();
     }
