@@ -237,6 +237,8 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
   __shared__ int s_pos[2][4];
   __shared__ T s_l11[W * W];              // strictly lower part of the diagonal block
   __shared__ T s_u[UB * LDU];             // U12 chunk, one column per thread
+  __shared__ T s_l21[RPT == 1 ? MAXM * LDU : 1];   // multipliers of the trailing rows (MFMA update)
+  __shared__ int s_prow[RPT == 1 ? MAXM : 1];      // their physical rows
   __shared__ int s_item[2];
   __shared__ int s_moved;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -368,6 +370,63 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
             if (k < w) { colp[s_perm[j0 + k]] = u[k]; s_u[t * LDU + k] = u[k]; }
         }
         __syncthreads();
+        if constexpr (RPT == 1) {
+          // A22 -= L21 U12 on MFMA 16x16 tiles (rows: positions >= j0+w at
+          // their physical rows, staged with their multipliers in LDS;
+          // 4 wavefronts, GT tiles per step with their C elements loaded
+          // before the products)
+          typedef BMF<T> mf;
+          const int r0 = j0 + w, nr = m - r0;
+          if (c0 == j0 + w && live[0] && pos[0] >= r0) {   // once per panel
+            s_prow[pos[0] - r0] = phys[0];
+#pragma unroll
+            for (int k = 0; k < W; k++) s_l21[(pos[0] - r0) * LDU + k] = k < w ? x[0][k] : T(0);
+          }
+          __syncthreads();
+          const int ntr = (nr + 15) / 16, ntc = (cw + 15) / 16, ntiles = ntr * ntc;
+          constexpr int GT = 4;
+          #pragma unroll 1
+          for (int tb = wave * GT; tb < ntiles; tb += 4 * GT) {
+            int ti[GT], tj[GT];
+            T cv[GT][4];
+            typename mf::acc acc[GT];
+#pragma unroll
+            for (int g = 0; g < GT; g++) {
+              const int tile = min(tb + g, ntiles - 1);
+              ti[g] = tile % ntr;
+              tj[g] = tile / ntr;
+              const int ri = ti[g] * 16 + (lane & 15);
+              const int pr = ri < nr ? s_prow[ri] : 0;
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int cc = tj[g] * 16 + mf::orow(lane, r);
+                cv[g][r] = (tb + g < ntiles && ri < nr && cc < cw) ? A[pr + (long)(c0 + cc) * lda] : T(0);
+              }
+            }
+#pragma unroll
+            for (int g = 0; g < GT; g++) {
+              acc[g] = typename mf::acc{T(0), T(0), T(0), T(0)};
+              const int ra = ti[g] * 16 + (lane & 15), cb = tj[g] * 16 + (lane & 15);
+#pragma unroll
+              for (int kk = 0; kk < W; kk += 4) {
+                const int k = kk + (lane >> 4);
+                const T fa = ra < nr ? s_l21[ra * LDU + k] : T(0);
+                const T fb = (cb < cw && k < w) ? s_u[cb * LDU + k] : T(0);
+                acc[g] = mf::mma(fb, fa, acc[g]);
+              }
+            }
+#pragma unroll
+            for (int g = 0; g < GT; g++) {
+              const int ri = ti[g] * 16 + (lane & 15);
+              const int pr = ri < nr ? s_prow[ri] : 0;
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int cc = tj[g] * 16 + mf::orow(lane, r);
+                if (tb + g < ntiles && ri < nr && cc < cw) A[pr + (long)(c0 + cc) * lda] = cv[g][r] - acc[g][r];
+              }
+            }
+          }
+        } else {
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
           if (!live[q] || pos[q] < j0 + w) continue;
@@ -391,6 +450,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
               }
             }
           }
+        }
         }
         __syncthreads();
       }
