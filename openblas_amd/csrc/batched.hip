@@ -286,12 +286,7 @@ __global__ __launch_bounds__(256) void bgetrf_kernel(const BDesc* __restrict__ d
         }
         double wv = bv;
         int wp = bp;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(wv, off);
-          const int op = __shfl_xor(wp, off);
-          if (better(ov, op, wv, wp)) { wv = ov; wp = op; }
-        }
+        lu_wave_best(wv, wp);   // DPP row moves + readlanes (no LDS shuffles)
         if (lane == 0) { s_key[buf][wave] = wv; s_pos[buf][wave] = wp; }
         if (bq >= 0 && bp == wp) {
 #pragma unroll

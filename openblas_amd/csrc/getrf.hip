@@ -139,37 +139,6 @@ __device__ inline V bypass_load(const V* p) {
 __device__ inline double lu_fms(double l, double u, double x) { return __builtin_fma(-l, u, x); }
 __device__ inline float lu_fms(float l, float u, float x) { return __builtin_fmaf(-l, u, x); }
 
-// one step of a wavefront max-reduction of (key, position) through DPP lane
-// moves (CTRL: a DPP control word whose source lanes are all in the row)
-template <int CTRL>
-__device__ __attribute__((always_inline)) inline void lu_dpp_best(double& bv, int& bp) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(bv), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(bv), CTRL, 0xF, 0xF, false);
-  const int op = __builtin_amdgcn_update_dpp(0, bp, CTRL, 0xF, 0xF, false);
-  const double ov = __hiloint2double(hi, lo);
-  if (better(ov, op, bv, bp)) { bv = ov; bp = op; }
-}
-
-// the wavefront's best (key, position), returned in every lane: butterfly
-// within each row of 16 lanes (quad_perm xor 1, xor 2, half-row mirror, row
-// mirror), then the four row results combined as uniform values
-__device__ __attribute__((always_inline)) inline void lu_wave_best(double& bv, int& bp) {
-  lu_dpp_best<0xB1>(bv, bp);
-  lu_dpp_best<0x4E>(bv, bp);
-  lu_dpp_best<0x141>(bv, bp);
-  lu_dpp_best<0x140>(bv, bp);
-  const int blo = __double2loint(bv), bhi = __double2hiint(bv);
-  double v = __hiloint2double(__builtin_amdgcn_readlane(bhi, 0), __builtin_amdgcn_readlane(blo, 0));
-  int p = __builtin_amdgcn_readlane(bp, 0);
-#pragma unroll
-  for (int r = 16; r < 64; r += 16) {
-    const double ov = __hiloint2double(__builtin_amdgcn_readlane(bhi, r), __builtin_amdgcn_readlane(blo, r));
-    const int op = __builtin_amdgcn_readlane(bp, r);
-    if (better(ov, op, v, p)) { v = ov; p = op; }
-  }
-  bv = v;
-  bp = p;
-}
 
 // x[k] -= l * u[k + OFF] for k in [K0, K1] with k <= kmax (kmax uniform;
 // u in LDS): whole chunks of 4 under one uniform branch, their 4 reads
