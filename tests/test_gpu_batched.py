@@ -275,6 +275,63 @@ def test_potrf_batch(oba, tag, uplo):
             assert np.array_equal(other(got, 1 if uplo == b"L" else -1), other(A0, 1 if uplo == b"L" else -1))
 
 
+@pytest.mark.parametrize("tag", ["d", "s"])
+def test_batched_small_sizes_mfma_updates(oba, tag):
+    """Batches whose largest problem has at most 256 rows run the one-row-
+    per-thread kernels, whose trailing updates are MFMA tiles (getrf: GEMM
+    tiles over the deferred-pivot physical rows, rectangular shapes
+    included; potrf: lower-triangle SYRK tiles, the other triangle
+    untouched): pivots and info as the oracle, factors to 100*n*eps."""
+    L = oba.lib()
+    dt = np.float64 if tag == "d" else np.float32
+    rng = np.random.default_rng(70 if tag == "d" else 71)
+    shapes = [(n, n) for n in sizes(rng, 30, 1, 256)] + [(256, 256), (200, 37), (37, 200), (256, 100), (17, 256)]
+    mats = make_lu_batch(rng, dt, shapes)
+    cnt = len(mats)
+    pool = DevPool(oba, mats)
+    pivs = [np.zeros(min(sh), np.int32) for sh in shapes]
+    ppool = DevPool(oba, pivs)
+    info = np.full(cnt, -99, np.int32)
+    m = iarr([sh[0] for sh in shapes]); n = iarr([sh[1] for sh in shapes])
+    f = L.openblas_amd_dgetrf_batch if tag == "d" else L.openblas_amd_sgetrf_batch
+    assert f(pi(m), pi(n), parr([pool.ptr(i) for i in range(cnt)]), pi(m), parr([ppool.ptr(i) for i in range(cnt)]),
+             pi(info), cnt) == 0
+    for i, A0 in enumerate(mats):
+        LU = pool.get(i, np.empty_like(A0))
+        piv = ppool.get(i, np.empty_like(pivs[i]))
+        mi, ni = shapes[i]
+        k = min(mi, ni)
+        LUo, pivo, infoo = oracle.getf2(A0)
+        assert info[i] == infoo == 0, (i, shapes[i])
+        if tag == "d":
+            assert np.array_equal(piv, pivo), (i, shapes[i])
+            assert np.abs(LU - LUo).max() <= 100 * max(mi, ni) * EPS[dt] * max(1, float(np.abs(LUo).max())), i
+        Lm = (np.tril(LU, -1)[:, :k] + np.eye(mi, k)).astype(np.float64)
+        Um = np.triu(LU)[:k, :].astype(np.float64)
+        PA = oracle.laswp(A0.astype(np.float64), 1, k, piv, 1)
+        assert np.linalg.norm(Lm @ Um - PA, 1) / (max(mi, ni) * np.linalg.norm(A0, 1) * EPS[dt]) < 30, i
+    for uplo in (b"L", b"U"):
+        ns = sizes(rng, 30, 1, 256) + [256]
+        spd = []
+        for nn in ns:
+            Bm = rng.standard_normal((nn, nn))
+            spd.append(np.asfortranarray((Bm @ Bm.T + nn * np.eye(nn)).astype(dt)))
+        pool = DevPool(oba, spd)
+        info = np.full(len(ns), -99, np.int32)
+        na = iarr(ns)
+        fp = L.openblas_amd_dpotrf_batch if tag == "d" else L.openblas_amd_spotrf_batch
+        assert fp(uplo, pi(na), parr([pool.ptr(i) for i in range(len(ns))]), pi(na), pi(info), len(ns)) == 0
+        for i, A0 in enumerate(spd):
+            got = pool.get(i, np.empty_like(A0))
+            Fo, infoo = oracle.potf2(1 if uplo == b"L" else 0, A0)
+            assert info[i] == infoo == 0, i
+            tri = np.tril if uplo == b"L" else np.triu
+            tol = 100 * ns[i] * EPS[dt] * max(1, float(np.abs(Fo).max()))
+            assert np.abs(tri(got) - tri(Fo)).max() <= tol, (uplo, i, ns[i])
+            other = np.triu if uplo == b"L" else np.tril
+            assert np.array_equal(other(got, 1 if uplo == b"L" else -1), other(A0, 1 if uplo == b"L" else -1))
+
+
 def test_batched_graph_replay(oba):
     """Same pointer table three times (graph replays), new contents each time,
     with other library work in between; every replay checked."""
