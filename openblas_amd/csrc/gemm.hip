@@ -122,7 +122,17 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
                                                       const T* __restrict__ A, long lda,
                                                       const T* __restrict__ B, long ldb, T beta,
                                                       T* __restrict__ C, long ldc, int tiles_m,
-                                                      int tiles_n, int tri, const int* abort_flag) {
+                                                      int tiles_n, int tri, const int* abort_flag,
+                                                      long kc, long cstride) {
+  // split K (gridDim.y parts, kc deep): part y multiplies its k range into
+  // its own C buffer (cstride apart); a second kernel sums the parts
+  if (kc > 0) {
+    const long y = blockIdx.y;
+    A += y * kc * (TA ? 1 : lda);
+    B += y * kc * (TB ? ldb : 1);
+    K = min(kc, K - y * kc);
+    C += y * cstride;
+  }
   constexpr int PAD = 64 / sizeof(T) * (sizeof(T) == 8 ? 2 : 1);  // 16 doubles / 16 floats
   constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
   constexpr int WTM = BM / 2, WTN = BN / 2;  // 2x2 waves
@@ -357,18 +367,32 @@ static void gemm_skinny(long m, long k, T alpha, const T* a, long lda, const T* 
 template <typename T, int BM, int BN, int BK, bool VEC>
 static void launch_cfg(bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
                        const T* B, long ldb, T beta, T* C, long ldc, hipStream_t s, int tri,
-                       const int* abort_flag) {
+                       const int* abort_flag, int ksplit = 1, long kc = 0, long cstride = 0) {
   long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   if (tm * tn > 0x7fffffffL) fatal("gemm: grid too large", nullptr);
-  dim3 grid((unsigned)(tm * tn)), block(256);
+  dim3 grid((unsigned)(tm * tn), (unsigned)ksplit), block(256);
 #define OBA_GEMM_L(TA_, TB_) \
-  gemm_kernel<T, BM, BN, BK, TA_, TB_, VEC><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int)tm, (int)tn, tri, abort_flag)
+  gemm_kernel<T, BM, BN, BK, TA_, TB_, VEC><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int)tm, (int)tn, tri, abort_flag, kc, cstride)
   if (!ta && !tb) OBA_GEMM_L(false, false);
   else if (!ta && tb) OBA_GEMM_L(false, true);
   else if (ta && !tb) OBA_GEMM_L(true, false);
   else OBA_GEMM_L(true, true);
 #undef OBA_GEMM_L
   OBA_HIP(hipGetLastError());
+}
+
+// C = sum of the ksplit partial products (alpha applied) + beta*C
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_ksum_kernel(long M, long N, int parts, const T* __restrict__ W, T beta,
+                                                        T* __restrict__ C, long ldc) {
+  const long total = M * N;
+  for (long idx = blockIdx.x * 256L + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
+    T sum = T(0);
+    for (int p = 0; p < parts; p++) sum += W[p * total + idx];
+    const long i = idx % M, j = idx / M;
+    T* cp = C + i + j * ldc;
+    *cp = beta == T(0) ? sum : sum + beta * *cp;   // beta == 0: C is not read
+  }
 }
 
 template <typename T>
@@ -398,6 +422,28 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
   // loaders need TILE*BK >= 256 vectors of 16 bytes
   constexpr int BK_BIG = sizeof(T) == 8 ? 4 : 8;
   constexpr int BK_SMALL = sizeof(T) == 8 ? 8 : 16;
+  const long tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
+  if (small && tri == 0 && abort_flag == nullptr && k >= 1024 && tiles64 <= rt().num_cu) {
+    // few tiles, deep K: split K so that ~4 workgroups per CU hide each
+    // other's latencies (1024^3 DGEMM: one 64x64 tile per CU otherwise)
+    const int ks = (int)std::min<long>({8L, std::max<long>(1L, 4L * rt().num_cu / tiles64), k / 256});
+    if (ks > 1) {
+      const long kc = ((k + ks - 1) / ks + 15) / 16 * 16;
+      const int parts = (int)((k + kc - 1) / kc);
+      DevBuf w(sizeof(T) * (size_t)parts * m * n);
+      if (vec)
+        launch_cfg<T, 64, 64, BK_SMALL, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, T(0), w.as<T>(), m, s, 0, nullptr,
+                                              parts, kc, m * n);
+      else
+        launch_cfg<T, 64, 64, BK_SMALL, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, T(0), w.as<T>(), m, s, 0,
+                                               nullptr, parts, kc, m * n);
+      const long total = m * n;
+      gemm_ksum_kernel<T><<<(unsigned)std::min<long>((total + 255) / 256, 4096), 256, 0, s>>>(m, n, parts, w.as<T>(),
+                                                                                             beta, c, ldc);
+      OBA_HIP(hipGetLastError());
+      return;
+    }
+  }
   if (small) {
     if (vec) launch_cfg<T, 64, 64, BK_SMALL, true>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);
     else launch_cfg<T, 64, 64, BK_SMALL, false>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, tri, abort_flag);

@@ -143,6 +143,35 @@ def test_gemm_few_columns(oba, dt):
         assert gemm_ratio(0, 0, 1.0, A[:m], B[:k], 0.0, np.zeros((m, n), dt), C[:m], dt) < 16, n
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_gemm_split_k(oba, dt):
+    """Few tiles with a deep K take the split-K path (partial products in a
+    workspace, summed with beta*C by a second kernel): dmmch ratio for all
+    four transpose forms, odd leading dimensions and a K tail, beta == 0
+    over NaN."""
+    L = oba.lib()
+    f = L.cblas_dgemm if dt == np.float64 else L.cblas_sgemm
+    rng = np.random.default_rng(23)
+    m, n, k = 300, 250, 3001
+    for ta in (0, 1):
+        for tb in (0, 1):
+            A = np.asfortranarray(rng.uniform(-1, 1, (k + 1, m + 1) if ta else (m + 1, k)).astype(dt))
+            B = np.asfortranarray(rng.uniform(-1, 1, (n + 1, k) if tb else (k + 1, n)).astype(dt))
+            C = np.asfortranarray(rng.uniform(-1, 1, (m + 3, n)).astype(dt))
+            C0 = C.copy(order="F")
+            f(102, TR[ta], TR[tb], m, n, k, 0.9, A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0], -0.4,
+              C.ctypes.data, m + 3)
+            Av = A[:k, :m] if ta else A[:m, :k]
+            Bv = B[:n, :k] if tb else B[:k, :n]
+            assert gemm_ratio(ta, tb, 0.9, Av, Bv, -0.4, C0[:m], C[:m], dt) < 16, (ta, tb)
+            assert np.array_equal(C[m:], C0[m:])   # rows beyond m untouched
+    A = np.asfortranarray(rng.uniform(-1, 1, (1024, 1024)).astype(dt))
+    C = np.full((1024, 1024), np.nan, dt, order="F")
+    f(102, 111, 111, 1024, 1024, 1024, 1.0, A.ctypes.data, 1024, A.ctypes.data, 1024, 0.0, C.ctypes.data, 1024)
+    assert not np.isnan(C).any()
+    assert gemm_ratio(0, 0, 1.0, A, A, 0.0, np.zeros_like(C), C, dt) < 16
+
+
 def test_fortran_gemm_matches_cblas(oba):
     L = oba.lib()
     rng = np.random.default_rng(12)
