@@ -64,6 +64,17 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
   }
   __syncthreads();
   if (t < TV_B) dinv[t] = (t < nr && !unit) ? T(1) / Md[t * LD + t] : T(1);
+  // wavefront 0's inputs of the diagonal solve (its own right-hand side, the
+  // diagonal block's row, the reciprocal pivot) do not depend on earlier
+  // blocks: fetched now, off the dependency chain
+  T v = T(0), myd = T(1);
+  T md[TV_B];
+  if (w == 0) {
+    v = lane < nr ? x[r0 + lane] : T(0);
+    myd = dinv[lane];   // written by this lane above
+#pragma unroll
+    for (int k = 0; k < TV_B; k++) md[k] = Md[lane * LD + k];
+  }
   T acc = T(0);
   T accv[16];
 #pragma unroll
@@ -136,7 +147,6 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
   }
   __syncthreads();
   if (w != 0 || bad) return;
-  T v = lane < nr ? x[r0 + lane] : T(0);
   if (ROWLANES) {
     v -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
   } else {
@@ -145,11 +155,7 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
     for (int l = 0; l < TV_B; l++) sacc[l & 3] += s_red[lane * LD + l];
     v -= (sacc[0] + sacc[1]) + (sacc[2] + sacc[3]);
   }
-  // the diagonal block's row of this lane in registers; static steps
-  T md[TV_B];
-#pragma unroll
-  for (int k = 0; k < TV_B; k++) md[k] = Md[lane * LD + k];
-  const T myd = dinv[lane];
+  // the diagonal block's row of this lane is in registers; static steps
   if (lower) {
 #pragma unroll
     for (int k = 0; k < TV_B; k++) {
