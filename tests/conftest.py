@@ -26,3 +26,28 @@ def oba():
     if openblas_amd.lib().openblas_amd_device_count() < 1:
         pytest.fail("no HIP device visible to libopenblas_amd")
     return openblas_amd
+
+
+@pytest.fixture(scope="session")
+def dsgesv_fallback_cases():
+    """Inputs that take dsgesv.f's two other fallback exits: an entry of B
+    or A beyond FLT_MAX (DLAG2S overflow, ITER = -2, dsgesv.f:291-305)
+    and an A whose single-precision copy has an all-zero column (entries
+    below FLT_TRUE_MIN demote to 0 without an overflow flag; SGETRF then
+    returns INFO > 0, ITER = -3, dsgesv.f:310-314)."""
+    rng = np.random.default_rng(23)
+    n = 40
+    base = rng.standard_normal((n, n)) + 10 * np.eye(n)
+    rhs = rng.standard_normal((n, 2))
+    cases = {}
+    B = rhs.copy()
+    B[3, 1] = 1e39
+    cases["B_overflow"] = (base.copy(), B, -2)
+    A = base.copy()
+    A[5, 7] = -1e39
+    A[7, 7] = 1e40
+    cases["A_overflow"] = (A, rhs.copy(), -2)
+    A = base.copy()
+    A[:, 4] *= 1e-50
+    cases["A_single_singular"] = (A, rhs.copy(), -3)
+    return cases

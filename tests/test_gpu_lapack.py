@@ -288,6 +288,29 @@ def test_dsgesv_golden_vectors(oba, golden):
                 assert np.array_equal(A, A0), "A must be unchanged after successful refinement"
 
 
+@pytest.mark.parametrize("name", ["B_overflow", "A_overflow", "A_single_singular"])
+def test_dsgesv_fallback_codes(oba, dsgesv_fallback_cases, name):
+    """ITER = -2 (a value beyond FLT_MAX) and -3 (single-precision LU
+    singular) take the FP64 DGETRF + DGETRS exit: ITER, INFO and pivots as
+    the oracle's, A overwritten by its factors, X to the same accuracy."""
+    L = oba.lib()
+    A0, B0, want = dsgesv_fallback_cases[name]
+    n, nrhs = B0.shape
+    A, B = np.asfortranarray(A0.copy()), np.asfortranarray(B0.copy())
+    X = np.zeros((n, nrhs), order="F")
+    piv = np.zeros(n, np.int32)
+    it, info = c_int(), c_int()
+    L.dsgesv_(oba.ip(n), oba.ip(nrhs), A.ctypes.data, oba.ip(n), piv.ctypes.data, B.ctypes.data, oba.ip(n),
+              X.ctypes.data, oba.ip(n), None, None, ctypes.byref(it), ctypes.byref(info))
+    Ao, pivo, Xo, ito, infoo = oracle.dsgesv(A0, B0)
+    assert it.value == ito == want and info.value == infoo == 0
+    assert np.array_equal(piv, pivo)
+    assert np.abs(A - Ao).max() <= 64 * n * EPS[np.float64] * np.abs(Ao).max()
+    assert np.all(np.abs(X - Xo).max(axis=0) <= 1e-12 * np.abs(Xo).max(axis=0))
+    bwd = np.abs(B0 - A0 @ X).max() / (np.abs(A0).sum(1).max() * np.abs(X).max())
+    assert bwd < 1e-15
+
+
 def test_lapacke_dsgesv_rowmajor(oba):
     L = oba.lib()
     rng = np.random.default_rng(51)

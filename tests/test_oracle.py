@@ -217,6 +217,21 @@ def test_dsgesv_live_reference_kappa2():
     assert np.max(np.abs(X - Xr)) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["B_overflow", "A_overflow", "A_single_singular"])
+def test_dsgesv_fallback_codes_match_reference(dsgesv_fallback_cases, name):
+    A, B, want = dsgesv_fallback_cases[name]
+    Xr, pivr, itr, infor = refblas.dsgesv(np.asfortranarray(A), np.asfortranarray(B))
+    Ao, pivo, X, it, info = oracle.dsgesv(A, B)
+    assert it == itr == want and info == infor == 0
+    # fallback = DGETRF + DGETRS on A: same pivots, factors in A, and a
+    # solution within a few ulps of the reference's
+    assert np.array_equal(pivo, pivr)
+    scale = np.abs(Xr).max(axis=0)
+    assert np.all(np.abs(X - Xr).max(axis=0) <= 1e-12 * scale)
+    bwd = np.abs(B - A @ X).max() / (np.abs(A).sum(1).max() * np.abs(X).max())
+    assert bwd < 1e-15
+
+
 def test_ctest_grid_harness_on_reference_library():
     """The din3 grid harness used by the GPU test, run against the reference
     library itself: pins the row-/column-major argument mapping of the
