@@ -17,7 +17,7 @@
 // layout), the split into k-contiguous int8 slices (64x64 tiles through
 // LDS), and one fused GEMM kernel: a workgroup owns a 64x64 tile of C, four
 // wavefronts own 32x32 each, all s slices of a 64-deep k slab of both
-// operands sit in LDS, every level's int32 accumulators and the FP64
+// operands sit in LDS (two slabs for s <= 9), every level's int32 accumulators and the FP64
 // accumulator stay in registers, and C is read and written once.
 #include "kernels.h"
 #include "ozaki.h"
@@ -128,17 +128,30 @@ __global__ __launch_bounds__(256) void oz_split_kernel(long V, long K, const dou
   }
 }
 
+// LDS byte offset of 16-byte chunk c (0..3) of row r of a slab tile.  With
+// two slab buffers (SW) rows are 64 bytes and the chunks are XOR-swizzled by
+// row bits 1 and 3 (stores: 4 lanes per row; MFMA fragment reads: 16 rows x
+// one chunk per 16 lanes; both touch every bank once per LDS cycle); with
+// one buffer rows are padded to 80 bytes.
+template <bool SW>
+__device__ __attribute__((always_inline)) inline int oz_lds_off(int r, int c) {
+  return SW ? r * 64 + ((c ^ (((r >> 1) ^ (r >> 3)) & 3)) << 4) : r * 80 + (c << 4);
+}
+
 // The fused slice GEMM.  LDS holds all s slices of a 64-deep slab for the
-// tile's 64 rows of op(A) and 64 columns of op(B), rows padded to 80 bytes.
+// tile's 64 rows of op(A) and 64 columns of op(B); for s <= 9 there are two
+// such buffers (147 KB), so the next slab is stored while this one is read
+// and each slab needs one barrier instead of two.
 template <int NS>
 __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, long Kp, const signed char* __restrict__ SA,
                                                       long Mp, const signed char* __restrict__ SB, long Np,
                                                       const double* __restrict__ amax, const double* __restrict__ bmax,
                                                       double alpha, double beta, double* __restrict__ C, long ldc,
                                                       long kchunk) {
-  constexpr int RS = 80;   // padded LDS row stride in bytes
-  __shared__ __attribute__((aligned(16))) signed char la[NS][OZ_T * RS];
-  __shared__ __attribute__((aligned(16))) signed char lb[NS][OZ_T * RS];
+  constexpr bool DB = NS <= 9;
+  constexpr int NBUF = DB ? 2 : 1, RS = DB ? 64 : 80;
+  __shared__ __attribute__((aligned(16))) signed char la[NBUF][NS][OZ_T * RS];
+  __shared__ __attribute__((aligned(16))) signed char lb[NBUF][NS][OZ_T * RS];
   // XCD-aware grouped tile order (workgroup b runs on XCD b%8): each XCD
   // takes a contiguous range of an order that walks 8 tile rows at a time,
   // so the slices of an A row block and a B column block are reused from
@@ -184,14 +197,26 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
       pa[t] = *(const i4v*)(ga + (size_t)t * Mp * Kp + kc0);
       pb[t] = *(const i4v*)(gb + (size_t)t * Np * Kp + kc0);
     }
-    for (long k0 = kc0; k0 < kc1; k0 += OZ_BK) {
-      // stage the slab: NS slices x 64 rows x 64 bytes for each operand
+    // stage a slab: NS slices x 64 rows x 64 bytes for each operand
+    const int soff = oz_lds_off<DB>(frow, fq >> 4);
+    int buf = 0;
+    if (DB) {
 #pragma unroll
       for (int t = 0; t < NS; t++) {
-        *(i4v*)(&la[t][frow * RS + fq]) = pa[t];
-        *(i4v*)(&lb[t][frow * RS + fq]) = pb[t];
+        *(i4v*)(&la[0][t][soff]) = pa[t];
+        *(i4v*)(&lb[0][t][soff]) = pb[t];
       }
       __syncthreads();
+    }
+    for (long k0 = kc0; k0 < kc1; k0 += OZ_BK) {
+      if (!DB) {
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          *(i4v*)(&la[0][t][soff]) = pa[t];
+          *(i4v*)(&lb[0][t][soff]) = pb[t];
+        }
+        __syncthreads();
+      }
       if (k0 + OZ_BK < kc1) {
 #pragma unroll
         for (int t = 0; t < NS; t++) {
@@ -204,12 +229,12 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
 #pragma unroll
       for (int u = 0; u < NS; u++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) fb[u][b] = *(const i4v*)(&lb[u][(wn + b * 16 + (lane & 15)) * RS + (lane >> 4) * 16]);
+        for (int b = 0; b < 2; b++) fb[u][b] = *(const i4v*)(&lb[buf][u][oz_lds_off<DB>(wn + b * 16 + (lane & 15), lane >> 4)]);
 #pragma unroll
       for (int t = 0; t < NS; t++) {
         i4v fa[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++) fa[a] = *(const i4v*)(&la[t][(wm + a * 16 + (lane & 15)) * RS + (lane >> 4) * 16]);
+        for (int a = 0; a < 2; a++) fa[a] = *(const i4v*)(&la[buf][t][oz_lds_off<DB>(wm + a * 16 + (lane & 15), lane >> 4)]);
 #pragma unroll
         for (int u = 0; u < NS - t; u++)
 #pragma unroll
@@ -217,6 +242,17 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
 #pragma unroll
             for (int b = 0; b < 2; b++)
               acc[t + u][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fb[u][b], fa[a], acc[t + u][a][b], 0, 0, 0);
+      }
+      if (DB) {
+        // the other buffer was last read before the previous barrier
+        if (k0 + OZ_BK < kc1) {
+#pragma unroll
+          for (int t = 0; t < NS; t++) {
+            *(i4v*)(&la[buf ^ 1][t][soff]) = pa[t];
+            *(i4v*)(&lb[buf ^ 1][t][soff]) = pb[t];
+          }
+        }
+        buf ^= 1;
       }
       __syncthreads();
     }
