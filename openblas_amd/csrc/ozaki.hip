@@ -96,35 +96,52 @@ __global__ __launch_bounds__(256) void oz_split_kernel(long V, long K, const dou
   __shared__ double tile[64][65];
   const long v0 = (long)blockIdx.x * 64, k0 = (long)blockIdx.y * 64;
   const bool vfast = (sv == 1);
-  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+  // all 16 loads of a thread in flight together, then the LDS stores
+  double ld[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int idx = (int)threadIdx.x + i * 256;
     const int f = idx & 63, g = idx >> 6;   // f: along the contiguous index
     const int vv = vfast ? f : g, kk = vfast ? g : f;
     const long v = v0 + vv, k = k0 + kk;
-    tile[vv][kk] = (v < V && k < K) ? X[v * sv + k * sk] : 0.0;
+    ld[i] = (v < V && k < K) ? X[v * sv + k * sk] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int idx = (int)threadIdx.x + i * 256;
+    const int f = idx & 63, g = idx >> 6;
+    tile[vfast ? f : g][vfast ? g : f] = ld[i];
   }
   __syncthreads();
   // each thread: one vector row of the tile, 16 consecutive k -> s x 16 bytes
   const int vv = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
   const long v = v0 + vv;
   const int e = (v < V) ? oz_exp(vmax[v]) : 0;
-  signed char d[OZ_SMAX][16];
+  double r[16];
 #pragma unroll
-  for (int c = 0; c < 16; c++) {
-    double r = ldexp(tile[vv][kq + c], -e);
+  for (int c = 0; c < 16; c++) r[c] = ldexp(tile[vv][kq + c], -e);
+  // digit t of every element, four digits per 32-bit word (byte c of the
+  // 16-byte store = element c); static indices keep r and the words in
+  // registers
 #pragma unroll
-    for (int t = 0; t < OZ_SMAX; t++) {
-      r *= 128.0;
-      const double q = trunc(r);
-      r -= q;
-      d[t][c] = (signed char)(int)q;
+  for (int t = 0; t < OZ_SMAX; t++) {
+    if (t < s) {
+      i4v w;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) {
+        unsigned word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int c = q4 * 4 + b;
+          r[c] *= 128.0;
+          const double q = trunc(r[c]);
+          r[c] -= q;
+          word |= ((unsigned)(int)q & 0xFFu) << (8 * b);
+        }
+        w[q4] = (int)word;
+      }
+      *(i4v*)(S + ((size_t)t * Vp + v) * Kp + k0 + kq) = w;
     }
-  }
-  for (int t = 0; t < s; t++) {
-    int4 w;
-    signed char* wb = (signed char*)&w;
-#pragma unroll
-    for (int c = 0; c < 16; c++) wb[c] = d[t][c];
-    *(int4*)(S + ((size_t)t * Vp + v) * Kp + k0 + kq) = w;
   }
 }
 
