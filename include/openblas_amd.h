@@ -11,8 +11,13 @@
  * otherwise) and may live in host memory or in device memory allocated with
  * hipMalloc/openblas_amd_malloc: host arguments are staged to the GPU and
  * back, device arguments are used in place.  Every call is synchronous.
- * blasint is a 32-bit int (LP64).  There is no CPU fallback: without a
- * gfx950 device the first call aborts with a message.
+ * blasint is a 32-bit int (LP64).  There is no CPU fallback.
+ *
+ * Device errors (no usable gfx950 device, a failed HIP call, a bounded
+ * in-kernel wait that ran out) never abort the caller: the call is skipped,
+ * LAPACK routines set INFO = OPENBLAS_AMD_DEVICE_ERROR, LAPACKE routines and
+ * the batched extensions return it, and openblas_amd_last_error() returns
+ * the message.  BLAS routines (void) leave their outputs unspecified then.
  */
 #ifndef OPENBLAS_AMD_H
 #define OPENBLAS_AMD_H
@@ -35,6 +40,10 @@ typedef enum CBLAS_SIDE      {CblasLeft = 141, CblasRight = 142} CBLAS_SIDE;
 
 #define LAPACK_ROW_MAJOR 101
 #define LAPACK_COL_MAJOR 102
+
+/* INFO / return value of a call that failed on the device (not an argument
+   position: no routine here has 1000 arguments) */
+#define OPENBLAS_AMD_DEVICE_ERROR (-1000)
 
 /* ---- CBLAS level 3 -------------------------------------------------------- */
 /* cblas.h:298-300, interface/gemm.c:294 (CNAME built with -DCBLAS) */
@@ -113,6 +122,10 @@ int slaswp_(const blasint *n, float *a, const blasint *lda, const blasint *k1, c
 int dsgesv_(const blasint *n, const blasint *nrhs, double *a, const blasint *lda, blasint *ipiv,
             const double *b, const blasint *ldb, double *x, const blasint *ldx, double *work,
             float *swork, blasint *iter, blasint *info);
+/* lapack-netlib/SRC/dsposv.f:197 (Cholesky-based; WORK and SWORK are accepted and not used) */
+int dsposv_(const char *uplo, const blasint *n, const blasint *nrhs, double *a, const blasint *lda,
+            const double *b, const blasint *ldb, double *x, const blasint *ldx, double *work,
+            float *swork, blasint *iter, blasint *info);
 
 /* ---- LAPACKE (lapack-netlib/LAPACKE/include/lapacke.h) ---------------------- */
 /* lapacke.h:872 and lapacke_dgesv_work.c:35 */
@@ -123,6 +136,14 @@ int LAPACKE_sgesv_work(int matrix_layout, int n, int nrhs, float *a, int lda, in
 /* lapacke.h:883 */
 int LAPACKE_dsgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb,
                    double *x, int ldx, int *iter);
+/* lapacke.h:5845 (WORK and SWORK are accepted and not used) */
+int LAPACKE_dsgesv_work(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb,
+                        double *x, int ldx, double *work, float *swork, int *iter);
+/* lapacke.h:2974 and lapacke.h:8477 */
+int LAPACKE_dsposv(int matrix_layout, char uplo, int n, int nrhs, double *a, int lda, double *b, int ldb,
+                   double *x, int ldx, int *iter);
+int LAPACKE_dsposv_work(int matrix_layout, char uplo, int n, int nrhs, double *a, int lda, double *b, int ldb,
+                        double *x, int ldx, double *work, float *swork, int *iter);
 /* lapacke.h:1062-1064 */
 int LAPACKE_dgetrf(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
 int LAPACKE_dgetrf_work(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
@@ -154,8 +175,13 @@ int openblas_get_num_procs(void);
 /* ---- extensions: no counterpart in the reference ---------------------------- */
 /* FP64 GEMM emulated with INT8 slices on the INT8 MFMA (Ozaki-style): the
    cblas_dgemm contract plus the slice count (0 = chosen from the operands'
-   exponent spread, else 1..12).  Returns the slice count used, -1 after an
-   argument error (xerbla "DGEMM ", info 15 for a bad slice count). */
+   exponent spread, else 1..12).  Returns the slice count used (0: nothing
+   to multiply, only beta*C ran); -1 after an argument error (xerbla
+   "DGEMM ", info 15 for a bad slice count); -2 when A or B holds a NaN or
+   an infinity (the native FP64 GEMM computed C, so they propagate as in
+   cblas_dgemm); -s (s > 12) when the automatic choice needs s slices, more
+   than the kernel holds (the native FP64 GEMM computed C, so the result
+   keeps FP64 accuracy); OPENBLAS_AMD_DEVICE_ERROR after a device error. */
 int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
                              blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
                              const double *B, blasint ldb, double beta, double *C, blasint ldc, int slices);
@@ -200,6 +226,9 @@ float openblas_amd_timer_stop(void);
 /* last argument error reported through xerbla on this thread (0 if none);
    copies the routine name; clears the record */
 int openblas_amd_last_xerbla(char *name, int len);
+/* 1 if a device error was recorded since the last call (0 if none); copies
+   its message; clears the record */
+int openblas_amd_last_error(char *msg, int len);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,12 @@ _SIGS = {
     "LAPACKE_sgesv": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p, _pd, c_int]),
     "LAPACKE_sgesv_work": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p, _pd, c_int]),
     "LAPACKE_dsgesv": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p, _pd, c_int, _pd, c_int, _pi]),
+    "LAPACKE_dsgesv_work": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p, _pd, c_int, _pd, c_int, c_void_p,
+                                    c_void_p, _pi]),
+    "dsposv_": (c_int, [c_char_p, _pi, _pi, _pd, _pi, _pd, _pi, _pd, _pi, c_void_p, c_void_p, _pi, _pi]),
+    "LAPACKE_dsposv": (c_int, [c_int, c_char, c_int, c_int, _pd, c_int, _pd, c_int, _pd, c_int, _pi]),
+    "LAPACKE_dsposv_work": (c_int, [c_int, c_char, c_int, c_int, _pd, c_int, _pd, c_int, _pd, c_int, c_void_p,
+                                    c_void_p, _pi]),
     "LAPACKE_dgetrf": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p]),
     "LAPACKE_dgetrf_work": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p]),
     "LAPACKE_sgetrf": (c_int, [c_int, c_int, c_int, _pd, c_int, c_void_p]),
@@ -99,6 +105,7 @@ _SIGS = {
     "openblas_amd_timer_start": (None, []),
     "openblas_amd_timer_stop": (c_float, []),
     "openblas_amd_last_xerbla": (c_int, [c_char_p, c_int]),
+    "openblas_amd_last_error": (c_int, [c_char_p, c_int]),
 }
 
 _lib = None

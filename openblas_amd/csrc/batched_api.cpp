@@ -143,8 +143,8 @@ int getrf_batch_impl(const char* name, const blasint* m, const blasint* n, T* co
     probs.push_back(d);
   }
   if (probs.empty()) return 0;
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Stager st;
   for (auto& d : probs) {
     st.add(&d.a, &d.lda, d.m, d.n, sizeof(T), true, true);
@@ -190,8 +190,8 @@ int getrs_batch_impl(const char* name, char trans, const blasint* n, const blasi
     probs.push_back(d);
   }
   if (probs.empty()) return 0;
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Stager st;
   for (auto& d : probs) {
     st.add(&d.a, &d.lda, d.n, d.n, sizeof(T), true, false);
@@ -234,8 +234,8 @@ int potrf_batch_impl(const char* name, char uplo, const blasint* n, T* const* a,
     probs.push_back(d);
   }
   if (probs.empty()) return 0;
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Stager st;
   for (auto& d : probs) st.add(&d.a, &d.lda, d.n, d.n, sizeof(T), true, true);
   st.finish();
@@ -255,7 +255,7 @@ template <typename T>
 void gemm_batch_run(std::vector<BDesc>& all) {
   if (all.empty()) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Stager st;
   for (auto& d : all) {
     const bool ta = d.flags & 1, tb = d.flags & 2;
@@ -291,27 +291,39 @@ using namespace oba;
 
 OBA_API int openblas_amd_dgetrf_batch(const blasint* m, const blasint* n, double* const* a, const blasint* lda,
                                       blasint* const* ipiv, blasint* info, blasint count) {
-  return getrf_batch_impl<double>("DGETRF", m, n, a, lda, ipiv, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = getrf_batch_impl<double>("DGETRF", m, n, a, lda, ipiv, info, count); });
+  return r;
 }
 OBA_API int openblas_amd_sgetrf_batch(const blasint* m, const blasint* n, float* const* a, const blasint* lda,
                                       blasint* const* ipiv, blasint* info, blasint count) {
-  return getrf_batch_impl<float>("SGETRF", m, n, a, lda, ipiv, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = getrf_batch_impl<float>("SGETRF", m, n, a, lda, ipiv, info, count); });
+  return r;
 }
 OBA_API int openblas_amd_dgetrs_batch(char trans, const blasint* n, const blasint* nrhs, double* const* a,
                                       const blasint* lda, blasint* const* ipiv, double* const* b, const blasint* ldb,
                                       blasint* info, blasint count) {
-  return getrs_batch_impl<double>("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = getrs_batch_impl<double>("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count); });
+  return r;
 }
 OBA_API int openblas_amd_sgetrs_batch(char trans, const blasint* n, const blasint* nrhs, float* const* a,
                                       const blasint* lda, blasint* const* ipiv, float* const* b, const blasint* ldb,
                                       blasint* info, blasint count) {
-  return getrs_batch_impl<float>("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = getrs_batch_impl<float>("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count); });
+  return r;
 }
 OBA_API int openblas_amd_dpotrf_batch(char uplo, const blasint* n, double* const* a, const blasint* lda, blasint* info,
                                       blasint count) {
-  return potrf_batch_impl<double>("DPOTRF", uplo, n, a, lda, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = potrf_batch_impl<double>("DPOTRF", uplo, n, a, lda, info, count); });
+  return r;
 }
 OBA_API int openblas_amd_spotrf_batch(char uplo, const blasint* n, float* const* a, const blasint* lda, blasint* info,
                                       blasint count) {
-  return potrf_batch_impl<float>("SPOTRF", uplo, n, a, lda, info, count);
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = potrf_batch_impl<float>("SPOTRF", uplo, n, a, lda, info, count); });
+  return r;
 }

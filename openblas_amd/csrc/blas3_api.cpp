@@ -39,7 +39,7 @@ void gemm_run(int transa, int transb, long m, long n, long k, T alpha, const T* 
   const bool only_scale = (k == 0 || alpha == T(0));
   if (only_scale && beta == T(1)) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Mat<T> A, B, C;
   if (!only_scale) {
     A.in(const_cast<T*>(a), transa ? k : m, transa ? m : k, lda, true);
@@ -109,7 +109,7 @@ void trsm_run(int side, int uplo, int trans, int unit, long m, long n, T alpha, 
               long ldb) {
   if (m == 0 || n == 0) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   long na = side ? n : m;
   Mat<T> A, B;
   B.in(b, m, n, ldb, alpha != T(0));
@@ -242,51 +242,51 @@ void gemm_batch_impl(const char* name, int order, const int* ta_arr, const int* 
 // ---- exported symbols ----------------------------------------------------------
 OBA_API void cblas_dgemm(int order, int ta, int tb, blasint m, blasint n, blasint k, double alpha, const double* a,
                          blasint lda, const double* b, blasint ldb, double beta, double* c, blasint ldc) {
-  cblas_gemm_impl<double>("DGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  guarded([&] { cblas_gemm_impl<double>("DGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
 }
 OBA_API void cblas_sgemm(int order, int ta, int tb, blasint m, blasint n, blasint k, float alpha, const float* a,
                          blasint lda, const float* b, blasint ldb, float beta, float* c, blasint ldc) {
-  cblas_gemm_impl<float>("SGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  guarded([&] { cblas_gemm_impl<float>("SGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
 }
 OBA_API void dgemm_(const char* ta, const char* tb, const blasint* m, const blasint* n, const blasint* k,
                     const double* alpha, const double* a, const blasint* lda, const double* b, const blasint* ldb,
                     const double* beta, double* c, const blasint* ldc) {
-  f_gemm_impl<double>("DGEMM ", ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  guarded([&] { f_gemm_impl<double>("DGEMM ", ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
 }
 OBA_API void sgemm_(const char* ta, const char* tb, const blasint* m, const blasint* n, const blasint* k,
                     const float* alpha, const float* a, const blasint* lda, const float* b, const blasint* ldb,
                     const float* beta, float* c, const blasint* ldc) {
-  f_gemm_impl<float>("SGEMM ", ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  guarded([&] { f_gemm_impl<float>("SGEMM ", ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
 }
 OBA_API void cblas_dtrsm(int order, int side, int uplo, int trans, int diag, blasint m, blasint n, double alpha,
                          const double* a, blasint lda, double* b, blasint ldb) {
-  cblas_trsm_impl<double>("DTRSM ", order, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+  guarded([&] { cblas_trsm_impl<double>("DTRSM ", order, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb); });
 }
 OBA_API void cblas_strsm(int order, int side, int uplo, int trans, int diag, blasint m, blasint n, float alpha,
                          const float* a, blasint lda, float* b, blasint ldb) {
-  cblas_trsm_impl<float>("STRSM ", order, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+  guarded([&] { cblas_trsm_impl<float>("STRSM ", order, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb); });
 }
 OBA_API void dtrsm_(const char* side, const char* uplo, const char* ta, const char* diag, const blasint* m,
                     const blasint* n, const double* alpha, const double* a, const blasint* lda, double* b,
                     const blasint* ldb) {
-  f_trsm_impl<double>("DTRSM ", side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb);
+  guarded([&] { f_trsm_impl<double>("DTRSM ", side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb); });
 }
 OBA_API void strsm_(const char* side, const char* uplo, const char* ta, const char* diag, const blasint* m,
                     const blasint* n, const float* alpha, const float* a, const blasint* lda, float* b,
                     const blasint* ldb) {
-  f_trsm_impl<float>("STRSM ", side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb);
+  guarded([&] { f_trsm_impl<float>("STRSM ", side, uplo, ta, diag, m, n, alpha, a, lda, b, ldb); });
 }
 OBA_API void cblas_dgemm_batch(int order, const int* ta, const int* tb, const blasint* m, const blasint* n,
                                const blasint* k, const double* alpha, const double** a, const blasint* lda,
                                const double** b, const blasint* ldb, const double* beta, double** c,
                                const blasint* ldc, blasint group_count, const blasint* group_size) {
-  gemm_batch_impl<double>("DGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
-                          group_size);
+  guarded([&] { gemm_batch_impl<double>("DGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
+                          group_size); });
 }
 OBA_API void cblas_sgemm_batch(int order, const int* ta, const int* tb, const blasint* m, const blasint* n,
                                const blasint* k, const float* alpha, const float** a, const blasint* lda,
                                const float** b, const blasint* ldb, const float* beta, float** c,
                                const blasint* ldc, blasint group_count, const blasint* group_size) {
-  gemm_batch_impl<float>("SGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
-                         group_size);
+  guarded([&] { gemm_batch_impl<float>("SGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
+                         group_size); });
 }

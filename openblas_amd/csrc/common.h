@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
+#include <stdexcept>
 
 typedef int blasint;
 
@@ -13,10 +14,40 @@ typedef int blasint;
 
 namespace oba {
 
-// ---- fatal / error plumbing ------------------------------------------------
-[[noreturn]] void fatal(const char* what, const char* detail);
+// ---- error plumbing ----------------------------------------------------------
+// A device-side failure (no usable device, a failed HIP call, a bounded wait
+// that ran out, an internal limit) never aborts the caller's process: it is
+// thrown as DeviceError, caught at the exported entry point (guarded()),
+// recorded for openblas_amd_last_error(), and reported to the caller as
+// INFO = OBA_DEVICE_ERROR (LAPACK/LAPACKE) or a skipped call (BLAS).
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+constexpr int OBA_DEVICE_ERROR = -1000;
+[[noreturn]] void fatal(const char* what, const char* detail);   // throws DeviceError
 void hip_check(hipError_t e, const char* what);
 #define OBA_HIP(x) ::oba::hip_check((x), #x)
+
+// Library calls are serialized by one recursive mutex (taken by guarded()).
+std::recursive_mutex& api_mutex();
+void note_error(const DeviceError& e);   // drains the device, records the message
+
+// Runs one exported call under the library lock; returns false (after
+// note_error) when it failed on the device.
+template <typename F>
+inline bool guarded(F&& f) noexcept {
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
+  try {
+    f();
+    return true;
+  } catch (const DeviceError& e) {
+    note_error(e);
+    return false;
+  } catch (const std::bad_alloc&) {
+    note_error(DeviceError("host allocation failed"));
+    return false;
+  }
+}
 
 // BLAS/LAPACK argument-error reporting (calls the weak, user-overridable
 // xerbla_ exactly like the reference, driver/others/xerbla.c:56-64).
@@ -31,7 +62,11 @@ struct Runtime {
   hipStream_t bulk = nullptr;     // like side, restricted to all but a few CUs
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   bool sync_on_return = true;     // synchronous semantics like the reference
-  std::recursive_mutex lock;      // library calls are serialized
+  // Sticky device error word in pinned, device-mapped host memory: kernels
+  // with a bounded wait set it (system-scope store) when the wait runs out;
+  // the host reads it after its synchronizations (check_device_error).
+  unsigned* d_err = nullptr;      // device address
+  volatile unsigned* h_err = nullptr;
 };
 Runtime& rt();                    // initializes on first use
 // Stream for throughput-bound work that runs beside latency-bound work on
@@ -41,7 +76,10 @@ Runtime& rt();                    // initializes on first use
 hipStream_t bulk_stream();
 
 // Device scratch pool (the analogue of blas_memory_alloc,
-// driver/others/memory.c:1161): cached blocks, reused across calls.
+// driver/others/memory.c:1161): cached blocks, reused across calls.  A freed
+// block is handed out again at once, so pool memory may only be used by work
+// on rt().stream (stream order then separates the users); code that runs on
+// another stream must not allocate from the pool.
 void* pool_alloc(size_t bytes);
 void pool_free(void* p);
 
@@ -76,6 +114,9 @@ struct Mat {
   void out();                                              // copy back
 };
 
-void end_call();   // synchronize if the runtime is in synchronous mode
+void end_call();   // synchronize if the runtime is in synchronous mode; check_device_error
+// throws DeviceError if a kernel has set the device error word (call after a
+// synchronization that covers the kernels in question)
+void check_device_error();
 
 }  // namespace oba

@@ -404,7 +404,13 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
     scale<T>(m, n, beta, c, ldc, s);
     return;
   }
-  if (!ta && !tb && n <= GS_NMAX && tri == 0 && abort_flag == nullptr && m >= 2048 && k >= 2048) {
+  // The split-K and few-column paths take a pool workspace.  Pool blocks are
+  // ordered on the library stream only (a block freed there may be handed
+  // out again before work queued on another stream has finished with it), so
+  // a GEMM on any other stream (the look-ahead LU's trailing update) uses
+  // the workspace-free tile kernels.
+  const bool ws_ok = s == rt().stream;
+  if (ws_ok && !ta && !tb && n <= GS_NMAX && tri == 0 && abort_flag == nullptr && m >= 2048 && k >= 2048) {
     switch (n) {
       case 1: gemm_skinny<T, 1>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
       case 2: gemm_skinny<T, 2>(m, k, alpha, a, lda, b, ldb, beta, c, ldc, s); return;
@@ -423,7 +429,7 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
   constexpr int BK_BIG = sizeof(T) == 8 ? 4 : 8;
   constexpr int BK_SMALL = sizeof(T) == 8 ? 8 : 16;
   const long tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
-  if (small && tri == 0 && abort_flag == nullptr && k >= 1024 && tiles64 <= rt().num_cu) {
+  if (ws_ok && small && tri == 0 && abort_flag == nullptr && k >= 1024 && tiles64 <= rt().num_cu) {
     // few tiles, deep K: split K so that ~4 workgroups per CU hide each
     // other's latencies (1024^3 DGEMM: one 64x64 tile per CU otherwise)
     const int ks = (int)std::min<long>({8L, std::max<long>(1L, 4L * rt().num_cu / tiles64), k / 256});

@@ -80,7 +80,7 @@ void getrf_impl(const char* name, const blasint* M, const blasint* N, T* a, cons
   *Info = 0;
   if (m == 0 || n == 0) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Mat<T> A;
   A.in(a, m, n, lda, true);
   IntDev P;
@@ -110,7 +110,7 @@ void getrs_impl(const char* name, const char* TRANS, const blasint* N, const bla
   *Info = 0;
   if (n == 0 || nrhs == 0) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Mat<T> A, B;
   A.in(const_cast<T*>(a), n, n, lda, true);
   B.in(b, n, nrhs, ldb, true);
@@ -135,7 +135,7 @@ void gesv_impl(const char* name, const blasint* N, const blasint* NRHS, T* a, co
   *Info = 0;
   if (n == 0 || nrhs == 0) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Mat<T> A, B;
   A.in(a, n, n, lda, true);
   B.in(b, n, nrhs, ldb, true);
@@ -164,7 +164,7 @@ void potrf_impl(const char* name, const char* UPLO, const blasint* N, T* a, cons
   *Info = 0;
   if (n == 0) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   Mat<T> A;
   A.in(a, n, n, lda, true);
   *Info = potrf<T>(uplo == 1, n, A.d, A.ld, r.stream);
@@ -180,7 +180,7 @@ void laswp_impl(const blasint* N, T* a, const blasint* LDA, const blasint* K1, c
   if (incx == 0 || n <= 0) return;   // interface/lapack/laswp.c:72
   if (k2 < k1) return;
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   const long inc = incx > 0 ? incx : -incx;
   const long npiv = (k1 - 1) + (k2 - k1) * inc + 1;
   std::vector<int> hp(npiv);
@@ -214,7 +214,7 @@ bool lapacke_nancheck_on() {
 }
 
 template <typename T>
-__global__ void nan_any_kernel(long m, long n, const T* a, long lda, int* flag);
+__global__ void nan_any_kernel(long m, long n, const T* a, long lda, int tri, int* flag);
 
 // A caller matrix that the LAPACK layer sees column-major on the device:
 // column-major host/device memory is staged as usual; row-major host memory
@@ -239,13 +239,15 @@ struct LMat {
     if (rowmajor && rows > 0 && cols > 0) transpose<T, T>(rows, cols, d, ld, raw.d, raw.ld, nullptr, rt().stream);
     raw.out();
   }
-  bool has_nan() {
+  // NaN anywhere (tri 0), or in the lower (1) / upper (2) triangle with the
+  // diagonal of the column-major view (LAPACKE_dtr_nancheck, diag 'N')
+  bool has_nan(int tri = 0) {
     if (rows <= 0 || cols <= 0) return false;
     DevBuf f(sizeof(int));
     OBA_HIP(hipMemsetAsync(f.p, 0, sizeof(int), rt().stream));
     long total = rows * cols;
     nan_any_kernel<T><<<(unsigned)std::min<long>((total + 255) / 256, 65535), 256, 0, rt().stream>>>(rows, cols, d, ld,
-                                                                                                      f.as<int>());
+                                                                                                      tri, f.as<int>());
     OBA_HIP(hipGetLastError());
     int h = 0;
     OBA_HIP(hipMemcpyAsync(&h, f.p, sizeof(int), hipMemcpyDeviceToHost, rt().stream));
@@ -258,10 +260,11 @@ struct LMat {
 
 namespace {
 template <typename T>
-__global__ void nan_any_kernel(long m, long n, const T* a, long lda, int* flag) {
+__global__ void nan_any_kernel(long m, long n, const T* a, long lda, int tri, int* flag) {
   long total = m * n;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     long i = idx % m, j = idx / m;
+    if ((tri == 1 && i < j) || (tri == 2 && i > j)) continue;
     if (isnan(a[i + j * lda])) { atomicOr(flag, 1); return; }
   }
 }
@@ -274,8 +277,8 @@ int lapacke_gesv(const char* name, const char* wname, int layout, int n, int nrh
     if (lda < n) { lapacke_xerbla(wname, -5); return -5; }
     if (ldb < nrhs) { lapacke_xerbla(wname, -8); return -8; }
   }
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   LMat<T> A, B;
   A.in(layout, a, n, n, lda, true);
   B.in(layout, b, n, nrhs, ldb, true);
@@ -297,8 +300,8 @@ int lapacke_getrf(const char* name, const char* wname, int layout, int m, int n,
                   bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR && lda < n) { lapacke_xerbla(wname, -5); return -5; }
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   LMat<T> A;
   A.in(layout, a, m, n, lda, true);
   if (nancheck && lapacke_nancheck_on() && A.has_nan()) return -5;
@@ -318,8 +321,8 @@ int lapacke_getrs(const char* name, const char* wname, int layout, char trans, i
     if (lda < n) { lapacke_xerbla(wname, -6); return -6; }
     if (ldb < nrhs) { lapacke_xerbla(wname, -9); return -9; }
   }
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   LMat<T> A, B;
   A.in(layout, const_cast<T*>(a), n, n, lda, true);
   B.in(layout, b, n, nrhs, ldb, true);
@@ -340,13 +343,16 @@ template <typename T>
 int lapacke_potrf(const char* name, const char* wname, int layout, char uplo, int n, T* a, int lda, bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR && lda < n) { lapacke_xerbla(wname, -5); return -5; }
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  (void)rt();
+  std::lock_guard<std::recursive_mutex> g(api_mutex());
   LMat<T> A;
   A.in(layout, a, n, n, lda, true);
-  // LAPACKE_dpo_nancheck looks at the referenced triangle only; this build
-  // does not perform the optional NaN pre-check for potrf.
-  (void)nancheck;
+  // LAPACKE_dpo_nancheck (lapacke_dpotrf.c:43-45): the referenced triangle
+  // only; after the row-major transpose the column-major view's uplo
+  // triangle is the referenced one.  A bad uplo skips the check (as
+  // LAPACKE_dtr_nancheck does) and potrf reports it.
+  const char uc = (char)toupper((unsigned char)uplo);
+  if (nancheck && lapacke_nancheck_on() && (uc == 'L' || uc == 'U') && A.has_nan(uc == 'L' ? 1 : 2)) return -4;
   blasint N = n, LDA = (blasint)std::max(1L, A.ld), info = 0;
   char u = uplo;
   potrf_impl<T>(Names<T>::potrf, &u, &N, A.d, &LDA, &info);
@@ -359,60 +365,73 @@ int lapacke_potrf(const char* name, const char* wname, int layout, char uplo, in
 
 // ---- Fortran LAPACK symbols ------------------------------------------------------
 OBA_API int dgetrf_(const blasint* m, const blasint* n, double* a, const blasint* lda, blasint* ipiv, blasint* info) {
-  getrf_impl<double>("DGETRF", m, n, a, lda, ipiv, info); return 0;
+  if (!guarded([&] { getrf_impl<double>("DGETRF", m, n, a, lda, ipiv, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int sgetrf_(const blasint* m, const blasint* n, float* a, const blasint* lda, blasint* ipiv, blasint* info) {
-  getrf_impl<float>("SGETRF", m, n, a, lda, ipiv, info); return 0;
+  if (!guarded([&] { getrf_impl<float>("SGETRF", m, n, a, lda, ipiv, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dgetf2_(const blasint* m, const blasint* n, double* a, const blasint* lda, blasint* ipiv, blasint* info) {
-  getrf_impl<double>("DGETF2", m, n, a, lda, ipiv, info); return 0;
+  if (!guarded([&] { getrf_impl<double>("DGETF2", m, n, a, lda, ipiv, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int sgetf2_(const blasint* m, const blasint* n, float* a, const blasint* lda, blasint* ipiv, blasint* info) {
-  getrf_impl<float>("SGETF2", m, n, a, lda, ipiv, info); return 0;
+  if (!guarded([&] { getrf_impl<float>("SGETF2", m, n, a, lda, ipiv, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dgetrs_(const char* trans, const blasint* n, const blasint* nrhs, const double* a, const blasint* lda,
                     const blasint* ipiv, double* b, const blasint* ldb, blasint* info) {
-  getrs_impl<double>("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info); return 0;
+  if (!guarded([&] { getrs_impl<double>("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int sgetrs_(const char* trans, const blasint* n, const blasint* nrhs, const float* a, const blasint* lda,
                     const blasint* ipiv, float* b, const blasint* ldb, blasint* info) {
-  getrs_impl<float>("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info); return 0;
+  if (!guarded([&] { getrs_impl<float>("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dgesv_(const blasint* n, const blasint* nrhs, double* a, const blasint* lda, blasint* ipiv, double* b,
                    const blasint* ldb, blasint* info) {
-  gesv_impl<double>("DGESV ", n, nrhs, a, lda, ipiv, b, ldb, info); return 0;
+  if (!guarded([&] { gesv_impl<double>("DGESV ", n, nrhs, a, lda, ipiv, b, ldb, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int sgesv_(const blasint* n, const blasint* nrhs, float* a, const blasint* lda, blasint* ipiv, float* b,
                    const blasint* ldb, blasint* info) {
-  gesv_impl<float>("SGESV ", n, nrhs, a, lda, ipiv, b, ldb, info); return 0;
+  if (!guarded([&] { gesv_impl<float>("SGESV ", n, nrhs, a, lda, ipiv, b, ldb, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dpotrf_(const char* uplo, const blasint* n, double* a, const blasint* lda, blasint* info) {
-  potrf_impl<double>("DPOTRF", uplo, n, a, lda, info); return 0;
+  if (!guarded([&] { potrf_impl<double>("DPOTRF", uplo, n, a, lda, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int spotrf_(const char* uplo, const blasint* n, float* a, const blasint* lda, blasint* info) {
-  potrf_impl<float>("SPOTRF", uplo, n, a, lda, info); return 0;
+  if (!guarded([&] { potrf_impl<float>("SPOTRF", uplo, n, a, lda, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dpotf2_(const char* uplo, const blasint* n, double* a, const blasint* lda, blasint* info) {
-  potrf_impl<double>("DPOTF2", uplo, n, a, lda, info); return 0;
+  if (!guarded([&] { potrf_impl<double>("DPOTF2", uplo, n, a, lda, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int spotf2_(const char* uplo, const blasint* n, float* a, const blasint* lda, blasint* info) {
-  potrf_impl<float>("SPOTF2", uplo, n, a, lda, info); return 0;
+  if (!guarded([&] { potrf_impl<float>("SPOTF2", uplo, n, a, lda, info); })) *info = OBA_DEVICE_ERROR;
+  return 0;
 }
 OBA_API int dlaswp_(const blasint* n, double* a, const blasint* lda, const blasint* k1, const blasint* k2,
                     const blasint* ipiv, const blasint* incx) {
-  laswp_impl<double>(n, a, lda, k1, k2, ipiv, incx); return 0;
+  guarded([&] { laswp_impl<double>(n, a, lda, k1, k2, ipiv, incx); });
+  return 0;
 }
 OBA_API int slaswp_(const blasint* n, float* a, const blasint* lda, const blasint* k1, const blasint* k2,
                     const blasint* ipiv, const blasint* incx) {
-  laswp_impl<float>(n, a, lda, k1, k2, ipiv, incx); return 0;
+  guarded([&] { laswp_impl<float>(n, a, lda, k1, k2, ipiv, incx); });
+  return 0;
 }
 
-// netlib dsgesv.f:193 signature; WORK and SWORK are accepted and unused (the
-// device pool holds the FP32 copy and the residual).
-OBA_API int dsgesv_(const blasint* N, const blasint* NRHS, double* a, const blasint* LDA, blasint* ipiv,
-                    const double* b, const blasint* LDB, double* x, const blasint* LDX, double* work, float* swork,
-                    blasint* ITER, blasint* INFO) {
-  (void)work; (void)swork;
+// netlib dsgesv.f:193 / dsposv.f:197 signatures; WORK and SWORK are accepted
+// and unused (the device pool holds the FP32 copy and the residual).
+namespace {
+void dsgesv_impl(const blasint* N, const blasint* NRHS, double* a, const blasint* LDA, blasint* ipiv, const double* b,
+                 const blasint* LDB, double* x, const blasint* LDX, blasint* ITER, blasint* INFO) {
   long n = *N, nrhs = *NRHS, lda = *LDA, ldb = *LDB, ldx = *LDX;
   blasint info = 0;
   *ITER = 0;
@@ -421,11 +440,10 @@ OBA_API int dsgesv_(const blasint* N, const blasint* NRHS, double* a, const blas
   else if (lda < std::max(1L, n)) info = -4;
   else if (ldb < std::max(1L, n)) info = -7;
   else if (ldx < std::max(1L, n)) info = -9;
-  if (info) { *INFO = info; report_xerbla("DSGESV", -info); return 0; }
+  if (info) { *INFO = info; report_xerbla("DSGESV", -info); return; }
   *INFO = 0;
-  if (n == 0) return 0;
+  if (n == 0) return;   // dsgesv.f:264; with NRHS = 0 the factorization still runs and sets IPIV
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
   Mat<double> A, B, X;
   A.in(a, n, n, lda, true);
   B.in(const_cast<double*>(b), n, nrhs, ldb, true);
@@ -433,96 +451,223 @@ OBA_API int dsgesv_(const blasint* N, const blasint* NRHS, double* a, const blas
   IntDev P;
   P.in(ipiv, n, false);
   int it = 0, inf = 0;
-  if (nrhs > 0) dsgesv_device(n, nrhs, A.d, A.ld, P.d, B.d, B.ld, X.d, X.ld, &it, &inf, nullptr, r.stream);
+  dsgesv_device(n, nrhs, A.d, A.ld, P.d, B.d, B.ld, X.d, X.ld, &it, &inf, nullptr, r.stream);
   *ITER = it;
   *INFO = inf;
   if (it < 0) A.out();   // A holds the FP64 factors only after the fallback
   X.out();
   P.out();
   end_call();
-  return 0;
 }
 
-// ---- LAPACKE symbols ----------------------------------------------------------------
-OBA_API int LAPACKE_dgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
-  return lapacke_gesv<double>("LAPACKE_dgesv", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true);
-}
-OBA_API int LAPACKE_dgesv_work(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
-  return lapacke_gesv<double>("LAPACKE_dgesv_work", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false);
-}
-OBA_API int LAPACKE_sgesv(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
-  return lapacke_gesv<float>("LAPACKE_sgesv", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true);
-}
-OBA_API int LAPACKE_sgesv_work(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
-  return lapacke_gesv<float>("LAPACKE_sgesv_work", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false);
-}
-OBA_API int LAPACKE_dgetrf(int layout, int m, int n, double* a, int lda, int* ipiv) {
-  return lapacke_getrf<double>("LAPACKE_dgetrf", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, true);
-}
-OBA_API int LAPACKE_dgetrf_work(int layout, int m, int n, double* a, int lda, int* ipiv) {
-  return lapacke_getrf<double>("LAPACKE_dgetrf_work", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, false);
-}
-OBA_API int LAPACKE_sgetrf(int layout, int m, int n, float* a, int lda, int* ipiv) {
-  return lapacke_getrf<float>("LAPACKE_sgetrf", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, true);
-}
-OBA_API int LAPACKE_sgetrf_work(int layout, int m, int n, float* a, int lda, int* ipiv) {
-  return lapacke_getrf<float>("LAPACKE_sgetrf_work", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, false);
-}
-OBA_API int LAPACKE_dgetrs(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b,
-                           int ldb) {
-  return lapacke_getrs<double>("LAPACKE_dgetrs", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true);
-}
-OBA_API int LAPACKE_dgetrs_work(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
-                                double* b, int ldb) {
-  return lapacke_getrs<double>("LAPACKE_dgetrs_work", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
-                               ldb, false);
-}
-OBA_API int LAPACKE_sgetrs(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b,
-                           int ldb) {
-  return lapacke_getrs<float>("LAPACKE_sgetrs", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true);
-}
-OBA_API int LAPACKE_sgetrs_work(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv,
-                                float* b, int ldb) {
-  return lapacke_getrs<float>("LAPACKE_sgetrs_work", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
-                              ldb, false);
-}
-OBA_API int LAPACKE_dpotrf(int layout, char uplo, int n, double* a, int lda) {
-  return lapacke_potrf<double>("LAPACKE_dpotrf", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, true);
-}
-OBA_API int LAPACKE_dpotrf_work(int layout, char uplo, int n, double* a, int lda) {
-  return lapacke_potrf<double>("LAPACKE_dpotrf_work", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, false);
-}
-OBA_API int LAPACKE_spotrf(int layout, char uplo, int n, float* a, int lda) {
-  return lapacke_potrf<float>("LAPACKE_spotrf", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, true);
-}
-OBA_API int LAPACKE_spotrf_work(int layout, char uplo, int n, float* a, int lda) {
-  return lapacke_potrf<float>("LAPACKE_spotrf_work", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, false);
-}
-OBA_API int LAPACKE_dsgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb, double* x,
-                           int ldx, int* iter) {
-  if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla("LAPACKE_dsgesv", -1); return -1; }
-  if (layout == LAPACK_ROW_MAJOR) {
-    if (lda < n) { lapacke_xerbla("LAPACKE_dsgesv_work", -5); return -5; }
-    if (ldb < nrhs) { lapacke_xerbla("LAPACKE_dsgesv_work", -8); return -8; }
-    if (ldx < nrhs) { lapacke_xerbla("LAPACKE_dsgesv_work", -10); return -10; }
-  }
+void dsposv_impl(const char* UPLO, const blasint* N, const blasint* NRHS, double* a, const blasint* LDA,
+                 const double* b, const blasint* LDB, double* x, const blasint* LDX, blasint* ITER, blasint* INFO) {
+  const char uc = (char)toupper((unsigned char)*UPLO);
+  long n = *N, nrhs = *NRHS, lda = *LDA, ldb = *LDB, ldx = *LDX;
+  blasint info = 0;
+  *ITER = 0;
+  if (uc != 'U' && uc != 'L') info = -1;
+  else if (n < 0) info = -2;
+  else if (nrhs < 0) info = -3;
+  else if (lda < std::max(1L, n)) info = -5;
+  else if (ldb < std::max(1L, n)) info = -7;
+  else if (ldx < std::max(1L, n)) info = -9;
+  if (info) { *INFO = info; report_xerbla("DSPOSV", -info); return; }
+  *INFO = 0;
+  if (n == 0) return;   // dsposv.f:257
   Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
+  Mat<double> A, B, X;
+  A.in(a, n, n, lda, true);
+  B.in(const_cast<double*>(b), n, nrhs, ldb, true);
+  X.in(x, n, nrhs, ldx, false);
+  int it = 0, inf = 0;
+  dsposv_device(uc == 'L', n, nrhs, A.d, A.ld, B.d, B.ld, X.d, X.ld, &it, &inf, nullptr, r.stream);
+  *ITER = it;
+  *INFO = inf;
+  if (it < 0) A.out();   // the FP64 Cholesky factor, only after the fallback
+  X.out();
+  end_call();
+}
+
+// LAPACKE_dsgesv_work / LAPACKE_dsposv_work (lapacke_dsgesv_work.c:35-110,
+// lapacke_dsposv_work.c:35-115): layout check, row-major transposes, info
+// shift; the NaN pre-check belongs to the high-level wrappers only.
+int lapacke_dsxsv(bool chol, char uplo, const char* name, const char* wname, int layout, int n, int nrhs, double* a,
+                  int lda, int* ipiv, double* b, int ldb, double* x, int ldx, int* iter, bool nancheck) {
+  if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
+  const int sh = chol ? 1 : 0;   // dsposv has UPLO in front
+  if (layout == LAPACK_ROW_MAJOR) {
+    if (lda < n) { lapacke_xerbla(wname, -5 - sh); return -5 - sh; }
+    if (ldb < nrhs) { lapacke_xerbla(wname, -8); return -8; }
+    if (ldx < nrhs) { lapacke_xerbla(wname, -10); return -10; }
+  }
   LMat<double> A, B, X;
   A.in(layout, a, n, n, lda, true);
   B.in(layout, b, n, nrhs, ldb, true);
   X.in(layout, x, n, nrhs, ldx, layout == LAPACK_ROW_MAJOR);
-  if (lapacke_nancheck_on()) {
-    if (A.has_nan()) return -4;
+  if (nancheck && lapacke_nancheck_on()) {
+    if (chol) {
+      const char uc = (char)toupper((unsigned char)uplo);
+      if ((uc == 'L' || uc == 'U') && A.has_nan(uc == 'L' ? 1 : 2)) return -5;
+    } else if (A.has_nan()) {
+      return -4;
+    }
     if (B.has_nan()) return -7;
   }
   blasint N = n, NRHS = nrhs, LDA = (blasint)std::max(1L, A.ld), LDB = (blasint)std::max(1L, B.ld),
           LDX = (blasint)std::max(1L, X.ld), it = 0, info = 0;
-  dsgesv_(&N, &NRHS, A.d, &LDA, ipiv, B.d, &LDB, X.d, &LDX, nullptr, nullptr, &it, &info);
+  if (chol) {
+    char u = uplo;
+    dsposv_impl(&u, &N, &NRHS, A.d, &LDA, B.d, &LDB, X.d, &LDX, &it, &info);
+  } else {
+    dsgesv_impl(&N, &NRHS, A.d, &LDA, ipiv, B.d, &LDB, X.d, &LDX, &it, &info);
+  }
   if (info < 0) info -= 1;
   *iter = it;
   if (it < 0) A.out();
   X.out();
   end_call();
   return info;
+}
+}  // namespace
+
+OBA_API int dsgesv_(const blasint* N, const blasint* NRHS, double* a, const blasint* LDA, blasint* ipiv,
+                    const double* b, const blasint* LDB, double* x, const blasint* LDX, double* work, float* swork,
+                    blasint* ITER, blasint* INFO) {
+  (void)work; (void)swork;
+  if (!guarded([&] { dsgesv_impl(N, NRHS, a, LDA, ipiv, b, LDB, x, LDX, ITER, INFO); })) *INFO = OBA_DEVICE_ERROR;
+  return 0;
+}
+OBA_API int dsposv_(const char* uplo, const blasint* N, const blasint* NRHS, double* a, const blasint* LDA,
+                    const double* b, const blasint* LDB, double* x, const blasint* LDX, double* work, float* swork,
+                    blasint* ITER, blasint* INFO) {
+  (void)work; (void)swork;
+  if (!guarded([&] { dsposv_impl(uplo, N, NRHS, a, LDA, b, LDB, x, LDX, ITER, INFO); })) *INFO = OBA_DEVICE_ERROR;
+  return 0;
+}
+
+// ---- LAPACKE symbols ----------------------------------------------------------------
+OBA_API int LAPACKE_dgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_gesv<double>("LAPACKE_dgesv", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true); });
+  return r;
+}
+OBA_API int LAPACKE_dgesv_work(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_gesv<double>("LAPACKE_dgesv_work", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false); });
+  return r;
+}
+OBA_API int LAPACKE_sgesv(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_gesv<float>("LAPACKE_sgesv", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true); });
+  return r;
+}
+OBA_API int LAPACKE_sgesv_work(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_gesv<float>("LAPACKE_sgesv_work", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false); });
+  return r;
+}
+OBA_API int LAPACKE_dgetrf(int layout, int m, int n, double* a, int lda, int* ipiv) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrf<double>("LAPACKE_dgetrf", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, true); });
+  return r;
+}
+OBA_API int LAPACKE_dgetrf_work(int layout, int m, int n, double* a, int lda, int* ipiv) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrf<double>("LAPACKE_dgetrf_work", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, false); });
+  return r;
+}
+OBA_API int LAPACKE_sgetrf(int layout, int m, int n, float* a, int lda, int* ipiv) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrf<float>("LAPACKE_sgetrf", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, true); });
+  return r;
+}
+OBA_API int LAPACKE_sgetrf_work(int layout, int m, int n, float* a, int lda, int* ipiv) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrf<float>("LAPACKE_sgetrf_work", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, false); });
+  return r;
+}
+OBA_API int LAPACKE_dgetrs(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b,
+                           int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrs<double>("LAPACKE_dgetrs", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true); });
+  return r;
+}
+OBA_API int LAPACKE_dgetrs_work(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
+                                double* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrs<double>("LAPACKE_dgetrs_work", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
+                               ldb, false); });
+  return r;
+}
+OBA_API int LAPACKE_sgetrs(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b,
+                           int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrs<float>("LAPACKE_sgetrs", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true); });
+  return r;
+}
+OBA_API int LAPACKE_sgetrs_work(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv,
+                                float* b, int ldb) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_getrs<float>("LAPACKE_sgetrs_work", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
+                              ldb, false); });
+  return r;
+}
+OBA_API int LAPACKE_dpotrf(int layout, char uplo, int n, double* a, int lda) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_potrf<double>("LAPACKE_dpotrf", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, true); });
+  return r;
+}
+OBA_API int LAPACKE_dpotrf_work(int layout, char uplo, int n, double* a, int lda) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_potrf<double>("LAPACKE_dpotrf_work", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, false); });
+  return r;
+}
+OBA_API int LAPACKE_spotrf(int layout, char uplo, int n, float* a, int lda) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_potrf<float>("LAPACKE_spotrf", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, true); });
+  return r;
+}
+OBA_API int LAPACKE_spotrf_work(int layout, char uplo, int n, float* a, int lda) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] { r = lapacke_potrf<float>("LAPACKE_spotrf_work", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, false); });
+  return r;
+}
+OBA_API int LAPACKE_dsgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb, double* x,
+                           int ldx, int* iter) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] {
+    r = lapacke_dsxsv(false, 'N', "LAPACKE_dsgesv", "LAPACKE_dsgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, x,
+                      ldx, iter, true);
+  });
+  return r;
+}
+OBA_API int LAPACKE_dsgesv_work(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb,
+                                double* x, int ldx, double* work, float* swork, int* iter) {
+  (void)work; (void)swork;
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] {
+    r = lapacke_dsxsv(false, 'N', "LAPACKE_dsgesv_work", "LAPACKE_dsgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb,
+                      x, ldx, iter, false);
+  });
+  return r;
+}
+OBA_API int LAPACKE_dsposv(int layout, char uplo, int n, int nrhs, double* a, int lda, double* b, int ldb, double* x,
+                           int ldx, int* iter) {
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] {
+    r = lapacke_dsxsv(true, uplo, "LAPACKE_dsposv", "LAPACKE_dsposv_work", layout, n, nrhs, a, lda, nullptr, b, ldb, x,
+                      ldx, iter, true);
+  });
+  return r;
+}
+OBA_API int LAPACKE_dsposv_work(int layout, char uplo, int n, int nrhs, double* a, int lda, double* b, int ldb,
+                                double* x, int ldx, double* work, float* swork, int* iter) {
+  (void)work; (void)swork;
+  int r = OBA_DEVICE_ERROR;
+  guarded([&] {
+    r = lapacke_dsxsv(true, uplo, "LAPACKE_dsposv_work", "LAPACKE_dsposv_work", layout, n, nrhs, a, lda, nullptr, b,
+                      ldb, x, ldx, iter, false);
+  });
+  return r;
 }

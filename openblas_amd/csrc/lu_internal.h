@@ -37,6 +37,14 @@ void getrs_perm(bool trans, long n, long nrhs, const T* a, long lda, const int* 
 template <typename T>
 void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, hipStream_t s);
 
+// The same on caller-provided workspace: `ticket` (one zeroed word) and `pub`
+// (trsv_lookback_ws_bytes(n) - 256 zeroed bytes); no allocation.
+template <typename T>
+size_t trsv_lookback_ws_bytes(long n);
+template <typename T>
+void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, unsigned long long* pub,
+                      unsigned* ticket, hipStream_t s);
+
 template <typename T>
 void getrs_t(long n, long nrhs, const T* At, long ldt, const int* perm_dev, T* b, long ldb, hipStream_t s);
 

@@ -22,6 +22,7 @@
 #include "kernels.h"
 #include "ozaki.h"
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <vector>
 
@@ -29,6 +30,7 @@ namespace oba {
 
 typedef int i4v __attribute__((ext_vector_type(4)));
 constexpr int OZ_SMAX = 12;     // slices supported by the fused kernel
+constexpr int OZ_NONFINITE = -2;   // return code: NaN/Inf operand, native GEMM used
 constexpr int OZ_BK = 64;       // k per slab (one i8 MFMA k-depth)
 constexpr int OZ_T = 64;        // C tile edge
 
@@ -36,16 +38,22 @@ constexpr int OZ_T = 64;        // C tile edge
 // sv == 1: the k range is cut into gridDim.y chunks (one vector per thread
 // and chunk, loads coalesced across the vectors); partial max/sum per chunk,
 // combined in chunk order by oz_stats_combine (deterministic)
-__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* pmax, double* psum) {
+// A NaN or infinity anywhere raises *nonfinite: the slices cannot carry it
+// (the host then runs the native FP64 GEMM, which propagates it).
+__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* pmax, double* psum,
+                                 int* nonfinite) {
   const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (v >= V) return;
   const long kc = (K + gridDim.y - 1) / gridDim.y, k0 = blockIdx.y * kc, k1 = min(K, k0 + kc);
   double mx = 0.0, sm = 0.0;
+  bool bad = false;
   for (long k = k0; k < k1; k++) {
     double a = fabs(X[v + k * sk]);
+    bad |= !(a <= DBL_MAX);
     mx = a > mx ? a : mx;
     sm += a;
   }
+  if (bad) atomicOr(nonfinite, 1);
   pmax[blockIdx.y * V + v] = mx;
   psum[blockIdx.y * V + v] = sm;
 }
@@ -64,16 +72,20 @@ __global__ void oz_stats_combine(long V, int parts, const double* __restrict__ p
   vsum[v] = sm;
 }
 
-__global__ void oz_stats_kcontig(long V, long K, const double* __restrict__ X, long sv, double* vmax, double* vsum) {
+__global__ void oz_stats_kcontig(long V, long K, const double* __restrict__ X, long sv, double* vmax, double* vsum,
+                                 int* nonfinite) {
   const long v = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6);   // sk == 1, one wave per vector
   const int lane = threadIdx.x & 63;
   if (v >= V) return;
   double mx = 0.0, sm = 0.0;
+  bool bad = false;
   for (long k = lane; k < K; k += 64) {
     double a = fabs(X[v * sv + k]);
+    bad |= !(a <= DBL_MAX);
     mx = a > mx ? a : mx;
     sm += a;
   }
+  if (bad) atomicOr(nonfinite, 1);
   for (int off = 32; off > 0; off >>= 1) {
     mx = fmax(mx, __shfl_xor(mx, off));
     sm += __shfl_xor(sm, off);
@@ -308,7 +320,8 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
   }
 }
 
-static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vsum, hipStream_t s) {
+static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vsum, int* nonfinite,
+                     hipStream_t s) {
   if (V <= 0) return;
   if (sv == 1) {
     // enough workgroups to fill the device: ~1024 (vector blocks x k chunks)
@@ -317,11 +330,11 @@ static void oz_stats(long V, long K, const double* X, long sv, long sk, double* 
     DevBuf part(sizeof(double) * 2 * (size_t)parts * V);
     double* pmax = part.as<double>();
     double* psum = pmax + (size_t)parts * V;
-    oz_stats_vcontig<<<dim3((unsigned)vb, (unsigned)parts), 256, 0, s>>>(V, K, X, sk, pmax, psum);
+    oz_stats_vcontig<<<dim3((unsigned)vb, (unsigned)parts), 256, 0, s>>>(V, K, X, sk, pmax, psum, nonfinite);
     OBA_HIP(hipGetLastError());
     oz_stats_combine<<<(unsigned)vb, 256, 0, s>>>(V, parts, pmax, psum, vmax, vsum);
   } else {
-    oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vsum);
+    oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vsum, nonfinite);
   }
   OBA_HIP(hipGetLastError());
 }
@@ -340,8 +353,8 @@ int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* 
     return worst;
   };
   const double bits = 53.0 + peak(M, amax, asum) + peak(N, bmax, bsum);
-  int s = (int)std::ceil(bits / 7.0);
-  return std::max(2, std::min(OZ_SMAX, s));
+  const double s = std::ceil(bits / 7.0);
+  return (int)std::max(2.0, std::min(s, 1000.0));   // the required count, not clamped to OZ_SMAX
 }
 
 int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const double* A, long lda, const double* B,
@@ -352,19 +365,34 @@ int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const do
   const long a_sv = ta ? lda : 1, a_sk = ta ? 1 : lda;
   // op(B) column j, element k: B[k + j*ldb] (N) or B[j + k*ldb] (T)
   const long b_sv = tb ? 1 : ldb, b_sk = tb ? ldb : 1;
-  DevBuf st(sizeof(double) * 2 * (M + N));
+  DevBuf st(sizeof(double) * 2 * (M + N) + 256);
   double* amax = st.as<double>();
   double* asum = amax + M;
   double* bmax = asum + M;
   double* bsum = bmax + N;
-  oz_stats(M, K, A, a_sv, a_sk, amax, asum, s);
-  oz_stats(N, K, B, b_sv, b_sk, bmax, bsum, s);
+  int* nonfinite = (int*)(bsum + N);
+  OBA_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
+  oz_stats(M, K, A, a_sv, a_sk, amax, asum, nonfinite, s);
+  oz_stats(N, K, B, b_sv, b_sk, bmax, bsum, nonfinite, s);
   int ns = slices;
+  int h_nonfinite = 0;
+  std::vector<double> h(ns <= 0 ? 2 * (M + N) : 0);
+  OBA_HIP(hipMemcpyAsync(&h_nonfinite, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (ns <= 0) OBA_HIP(hipMemcpyAsync(h.data(), amax, sizeof(double) * 2 * (M + N), hipMemcpyDeviceToHost, s));
+  OBA_HIP(hipStreamSynchronize(s));
+  if (h_nonfinite) {
+    // NaN / Inf: the native FP64 GEMM gives cblas_dgemm's propagation
+    gemm<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return OZ_NONFINITE;
+  }
   if (ns <= 0) {
-    std::vector<double> h(2 * (M + N));
-    OBA_HIP(hipMemcpyAsync(h.data(), amax, sizeof(double) * 2 * (M + N), hipMemcpyDeviceToHost, s));
-    OBA_HIP(hipStreamSynchronize(s));
     ns = ozaki_slices_auto(M, N, K, h.data(), h.data() + M, h.data() + 2 * M, h.data() + 2 * M + N);
+    if (ns > OZ_SMAX) {
+      // the exponent spread needs more digits than the fused kernel holds: a
+      // 12-slice result would miss FP64 accuracy, so the native FP64 GEMM runs
+      gemm<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+      return -ns;
+    }
   }
   if (ns < 1 || ns > OZ_SMAX) fatal("ozaki: slice count out of range", nullptr);
   const long Kp = (K + OZ_BK - 1) / OZ_BK * OZ_BK;
@@ -400,7 +428,10 @@ using namespace oba;
 
 // Extension entry point (no reference counterpart): cblas_dgemm semantics
 // plus the slice count (0 = chosen from the operands' exponent spread).
-// Returns the slice count used, or -1 after an argument error.
+// Returns the slice count used (0: nothing to multiply), -1 after an argument
+// error, OZ_NONFINITE (-2) when an operand holds a NaN or infinity and the
+// native FP64 GEMM ran, -s (s > 12) when the operands need s slices and the
+// native FP64 GEMM ran, OBA_DEVICE_ERROR after a device error.
 OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, blasint m, blasint n, blasint k, double alpha,
                                      const double* a, blasint lda, const double* b, blasint ldb, double beta,
                                      double* c, blasint ldc, int slices) {
@@ -426,14 +457,20 @@ OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, blasint 
   if (info >= 0) { report_xerbla("DGEMM ", info); return -1; }
   if (slices < 0 || slices > OZ_SMAX) { report_xerbla("DGEMM ", 15); return -1; }
   if (M == 0 || N == 0) return 0;
-  Runtime& r = rt();
-  std::lock_guard<std::recursive_mutex> g(r.lock);
-  Mat<double> A, B, C;
-  A.in(const_cast<double*>(pa), transa ? K : M, transa ? M : K, LDA, true);
-  B.in(const_cast<double*>(pb), transb ? N : K, transb ? K : N, LDB, true);
-  C.in(c, M, N, ldc, beta != 0.0);
-  int ns = ozaki_dgemm(transa == 1, transb == 1, M, N, K, alpha, A.d, A.ld, B.d, B.ld, beta, C.d, C.ld, slices, r.stream);
-  C.out();
-  end_call();
+  int ns = OBA_DEVICE_ERROR;
+  guarded([&] {
+    Runtime& r = rt();
+    Mat<double> A, B, C;
+    const bool mul = K > 0 && alpha != 0.0;
+    if (mul) {
+      A.in(const_cast<double*>(pa), transa ? K : M, transa ? M : K, LDA, true);
+      B.in(const_cast<double*>(pb), transb ? N : K, transb ? K : N, LDB, true);
+    }
+    C.in(c, M, N, ldc, beta != 0.0);
+    ns = ozaki_dgemm(transa == 1, transb == 1, M, N, K, alpha, A.d, A.ld, B.d, B.ld, beta, C.d, C.ld, slices,
+                     r.stream);
+    C.out();
+    end_call();
+  });
   return ns;
 }

@@ -6,7 +6,7 @@
 // driver/others/xerbla.c:56-64 (weak xerbla_), driver/others/openblas_env.c
 // (environment), interface/... thread-count knobs (openblas_set_num_threads,
 // cblas.h:13-24).  There is no CPU fallback anywhere: a missing device is a
-// fatal error.
+// device error reported to the caller (DeviceError, caught per entry point).
 #include "common.h"
 #include <cstdio>
 #include <cstdlib>
@@ -18,18 +18,42 @@
 
 namespace oba {
 
+static Runtime* g_rt = nullptr;
+static std::once_flag g_rt_once;
+
 [[noreturn]] void fatal(const char* what, const char* detail) {
-  fprintf(stderr, "libopenblas_amd: fatal: %s%s%s\n", what, detail ? ": " : "", detail ? detail : "");
+  std::string m = what;
+  if (detail) { m += ": "; m += detail; }
+  throw DeviceError(m);
+}
+
+std::recursive_mutex& api_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
+static std::mutex g_err_mu;
+static std::string g_last_error;
+
+void note_error(const DeviceError& e) {
+  {
+    std::lock_guard<std::mutex> g(g_err_mu);
+    g_last_error = e.what();
+  }
+  fprintf(stderr, "libopenblas_amd: device error: %s (call skipped)\n", e.what());
   fflush(stderr);
-  abort();
+  // nothing the failed call queued may still run on buffers that go back to
+  // the pool; every wait in the library's kernels is bounded, so this ends
+  int n = 0;
+  if (hipGetDeviceCount(&n) == hipSuccess && n > 0) (void)hipDeviceSynchronize();
+  (void)hipGetLastError();
+  if (g_rt && g_rt->h_err) *g_rt->h_err = 0u;
 }
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fatal(what, hipGetErrorString(e));
 }
 
-static Runtime* g_rt = nullptr;
-static std::once_flag g_rt_once;
 
 Runtime& rt() {
   std::call_once(g_rt_once, [] {
@@ -52,6 +76,13 @@ Runtime& rt() {
     OBA_HIP(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
     OBA_HIP(hipEventCreateWithFlags(&r->ev_a, hipEventDisableTiming));
     OBA_HIP(hipEventCreateWithFlags(&r->ev_b, hipEventDisableTiming));
+    void* he = nullptr;
+    OBA_HIP(hipHostMalloc(&he, 256, hipHostMallocMapped));
+    void* de = nullptr;
+    OBA_HIP(hipHostGetDevicePointer(&de, he, 0));
+    r->h_err = (volatile unsigned*)he;
+    r->d_err = (unsigned*)de;
+    *r->h_err = 0u;
     if (const char* s = getenv("OPENBLAS_AMD_ASYNC")) r->sync_on_return = (atoi(s) == 0);
     g_rt = r;
   });
@@ -94,9 +125,22 @@ hipStream_t bulk_stream() {
   return r.bulk;
 }
 
+void check_device_error() {
+  Runtime& r = rt();
+  const unsigned e = *r.h_err;
+  if (e) {
+    *r.h_err = 0u;
+    throw DeviceError(e == 1u ? "a cross-workgroup wait ran out (kernel exchange timed out)"
+                              : "a kernel found inconsistent exchange data");
+  }
+}
+
 void end_call() {
   Runtime& r = rt();
-  if (r.sync_on_return) OBA_HIP(hipStreamSynchronize(r.stream));
+  if (r.sync_on_return) {
+    OBA_HIP(hipStreamSynchronize(r.stream));
+    check_device_error();
+  }
 }
 
 // ---- device pool -----------------------------------------------------------
@@ -136,7 +180,7 @@ void* pool_alloc(size_t bytes) {
     g_free.clear();
     g_cached_bytes = 0;
     e = hipMalloc(&p, sz);
-    if (e != hipSuccess) fatal("device allocation failed", hipGetErrorString(e));
+    if (e != hipSuccess) { (void)hipGetLastError(); fatal("device allocation failed", hipGetErrorString(e)); }
   }
   std::lock_guard<std::mutex> g(g_pool_mu);
   g_size[p] = sz;
@@ -147,7 +191,7 @@ void pool_free(void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> g(g_pool_mu);
   auto it = g_size.find(p);
-  if (it == g_size.end()) fatal("pool_free of unknown block", nullptr);
+  if (it == g_size.end()) return;   // not a pool block (cannot happen; never throw from a destructor)
   g_free.emplace(it->second, p);
   g_cached_bytes += it->second;
 }
@@ -229,59 +273,87 @@ OBA_API char* openblas_get_config(void) {
 OBA_API char* openblas_get_corename(void) { static char n[] = "gfx950"; return n; }
 OBA_API void openblas_set_num_threads(int n) { g_threads = n > 0 ? n : 1; }
 OBA_API int openblas_get_num_threads(void) { return g_threads; }
-OBA_API int openblas_get_num_procs(void) { return rt().num_cu; }
+OBA_API int openblas_get_num_procs(void) {
+  int n = 0;
+  guarded([&] { n = rt().num_cu; });
+  return n;
+}
 
 // ---- extension API (no reference counterpart; used by tests/benchmarks) ----
 OBA_API void* openblas_amd_malloc(size_t bytes) {
-  rt();
   void* p = nullptr;
-  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  guarded([&] {
+    rt();
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) { (void)hipGetLastError(); p = nullptr; }
+  });
   return p;
 }
-OBA_API void openblas_amd_free(void* p) { if (p) { rt(); OBA_HIP(hipFree(p)); } }
+OBA_API void openblas_amd_free(void* p) {
+  if (p) guarded([&] { rt(); OBA_HIP(hipFree(p)); });
+}
 OBA_API int openblas_amd_memcpy(void* dst, const void* src, size_t bytes, int kind) {
-  Runtime& r = rt();
-  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  OBA_HIP(hipMemcpyAsync(dst, src, bytes, k, r.stream));
-  OBA_HIP(hipStreamSynchronize(r.stream));
-  return 0;
+  return guarded([&] {
+    Runtime& r = rt();
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    OBA_HIP(hipMemcpyAsync(dst, src, bytes, k, r.stream));
+    OBA_HIP(hipStreamSynchronize(r.stream));
+  }) ? 0 : OBA_DEVICE_ERROR;
 }
 OBA_API int openblas_amd_memset(void* dst, int value, size_t bytes) {
-  Runtime& r = rt();
-  OBA_HIP(hipMemsetAsync(dst, value, bytes, r.stream));
-  OBA_HIP(hipStreamSynchronize(r.stream));
-  return 0;
+  return guarded([&] {
+    Runtime& r = rt();
+    OBA_HIP(hipMemsetAsync(dst, value, bytes, r.stream));
+    OBA_HIP(hipStreamSynchronize(r.stream));
+  }) ? 0 : OBA_DEVICE_ERROR;
 }
-OBA_API int openblas_amd_synchronize(void) { OBA_HIP(hipStreamSynchronize(rt().stream)); return 0; }
-OBA_API void openblas_amd_set_sync(int on) { rt().sync_on_return = on != 0; }
+OBA_API int openblas_amd_synchronize(void) {
+  return guarded([] {
+    OBA_HIP(hipStreamSynchronize(rt().stream));
+    check_device_error();
+  }) ? 0 : OBA_DEVICE_ERROR;
+}
+OBA_API void openblas_amd_set_sync(int on) { guarded([on] { rt().sync_on_return = on != 0; }); }
+// message of the last device error (empty if none); clears it
+OBA_API int openblas_amd_last_error(char* msg, int len) {
+  std::lock_guard<std::mutex> g(g_err_mu);
+  const int had = g_last_error.empty() ? 0 : 1;
+  if (msg && len > 0) { strncpy(msg, g_last_error.c_str(), len - 1); msg[len - 1] = 0; }
+  g_last_error.clear();
+  return had;
+}
 OBA_API int openblas_amd_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
   return n;
 }
 OBA_API int openblas_amd_device_info(char* name, int len, int* num_cu, int* clock_khz) {
-  Runtime& r = rt();
-  hipDeviceProp_t p;
-  OBA_HIP(hipGetDeviceProperties(&p, r.device));
-  if (name && len > 0) { strncpy(name, p.gcnArchName, len - 1); name[len - 1] = 0; }
-  if (num_cu) *num_cu = p.multiProcessorCount;
-  if (clock_khz) *clock_khz = p.clockRate;
-  return 0;
+  return guarded([&] {
+    Runtime& r = rt();
+    hipDeviceProp_t p;
+    OBA_HIP(hipGetDeviceProperties(&p, r.device));
+    if (name && len > 0) { strncpy(name, p.gcnArchName, len - 1); name[len - 1] = 0; }
+    if (num_cu) *num_cu = p.multiProcessorCount;
+    if (clock_khz) *clock_khz = p.clockRate;
+  }) ? 0 : OBA_DEVICE_ERROR;
 }
 // HIP-event timer on the library stream: every kernel the library launches
 // goes to this stream, so start/stop bracket exactly the library's work.
 static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
 OBA_API void openblas_amd_timer_start(void) {
-  Runtime& r = rt();
-  if (!g_t0) { OBA_HIP(hipEventCreate(&g_t0)); OBA_HIP(hipEventCreate(&g_t1)); }
-  OBA_HIP(hipEventRecord(g_t0, r.stream));
+  guarded([] {
+    Runtime& r = rt();
+    if (!g_t0) { OBA_HIP(hipEventCreate(&g_t0)); OBA_HIP(hipEventCreate(&g_t1)); }
+    OBA_HIP(hipEventRecord(g_t0, r.stream));
+  });
 }
 OBA_API float openblas_amd_timer_stop(void) {
-  Runtime& r = rt();
-  OBA_HIP(hipEventRecord(g_t1, r.stream));
-  OBA_HIP(hipEventSynchronize(g_t1));
-  float ms = 0.f;
-  OBA_HIP(hipEventElapsedTime(&ms, g_t0, g_t1));
+  float ms = -1.f;
+  guarded([&] {
+    Runtime& r = rt();
+    OBA_HIP(hipEventRecord(g_t1, r.stream));
+    OBA_HIP(hipEventSynchronize(g_t1));
+    OBA_HIP(hipEventElapsedTime(&ms, g_t0, g_t1));
+  });
   return ms;
 }
 OBA_API int openblas_amd_last_xerbla(char* name, int len) {

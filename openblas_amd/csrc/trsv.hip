@@ -117,8 +117,8 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
       __builtin_amdgcn_s_sleep(1);   // back off: every block's wavefronts poll
       ++spins;
       if (spins > (1L << 22) ||
-          ((spins & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        if (lane == 0) atomicExch(err, 1u);
+          ((spins & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))) {
+        if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         ok = false;
         break;
       }
@@ -197,20 +197,40 @@ void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, 
   const size_t pub_bytes = sizeof(unsigned long long) * (size_t)nblk * TV_B * TvPub<T>::WPV;
   DevBuf ws(256 + pub_bytes);
   unsigned* ticket = ws.as<unsigned>();
-  unsigned* err = ticket + 16;
   unsigned long long* pub = (unsigned long long*)(ws.as<char>() + 256);
   OBA_HIP(hipMemsetAsync(ws.p, 0, 256 + pub_bytes, s));
+  trsv_lookback_ws<T>(lower, unit, n, a, rs, cs, x, pub, ticket, s);
+}
+
+template <typename T>
+size_t trsv_lookback_ws_bytes(long n) {
+  return 256 + sizeof(unsigned long long) * (size_t)((n + TV_B - 1) / TV_B) * TV_B * TvPub<T>::WPV;
+}
+
+// The launch on caller-provided, zeroed workspace (ticket: 1 word, pub:
+// trsv_lookback_ws_bytes(n) - 256 bytes).  A wait that runs out sets the
+// runtime's device error word, which the API layer checks after its next
+// synchronization (the kernel itself never blocks the host).
+template <typename T>
+void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, unsigned long long* pub,
+                      unsigned* ticket, hipStream_t s) {
+  if (n <= 0) return;
+  if (rs != 1 && cs != 1) fatal("trsv_lookback: one index of A must be contiguous", nullptr);
+  const long nblk = (n + TV_B - 1) / TV_B;
+  unsigned* err = rt().d_err;
   if (rs == 1)
     trsv_lb_kernel<T, true><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
   else
     trsv_lb_kernel<T, false><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
   OBA_HIP(hipGetLastError());
-  unsigned h = 0;
-  OBA_HIP(hipMemcpyAsync(&h, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  OBA_HIP(hipStreamSynchronize(s));
-  if (h) fatal("trsv: block exchange timed out", nullptr);
 }
 
+template size_t trsv_lookback_ws_bytes<double>(long);
+template size_t trsv_lookback_ws_bytes<float>(long);
+template void trsv_lookback_ws<double>(bool, bool, long, const double*, long, long, double*, unsigned long long*,
+                                       unsigned*, hipStream_t);
+template void trsv_lookback_ws<float>(bool, bool, long, const float*, long, long, float*, unsigned long long*,
+                                      unsigned*, hipStream_t);
 template void trsv_lookback<double>(bool, bool, long, const double*, long, long, double*, hipStream_t);
 template void trsv_lookback<float>(bool, bool, long, const float*, long, long, float*, hipStream_t);
 

@@ -45,6 +45,8 @@ def lib():
             "or_dpotf2": (c_int, [c_int, c_int, P, c_int]),
             "or_spotf2": (c_int, [c_int, c_int, P, c_int]),
             "or_dsgesv": (c_int, [c_int, c_int, P, c_int, P, P, c_int, P, c_int, P]),
+            "or_dsposv": (c_int, [c_int, c_int, c_int, P, c_int, P, c_int, P, c_int, P]),
+            "or_dpotrs": (None, [c_int, c_int, c_int, P, c_int, P, c_int]),
             "or_dmmch_ratio": (c_double, [c_int, c_int, c_int, c_int, c_int, c_double, P, c_int, P, c_int, c_double,
                                           P, c_int, P, c_int, c_double]),
             "or_dbeg": (c_double, [c_int]),
@@ -136,6 +138,25 @@ def dsgesv(A, B):
     info = lib().or_dsgesv(n, nrhs, _p(A), max(n, 1), _p(ipiv), _p(B), max(n, 1), _p(X), max(n, 1),
                            ctypes.addressof(it))
     return A, ipiv[:n], X, it.value, info
+
+
+def dsposv(uplo, A, B):
+    """uplo 1 lower / 0 upper.  Returns (A_out, X, iter, info)."""
+    A, B = _f(A, np.float64), _f(B, np.float64)
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    it = c_int(0)
+    info = lib().or_dsposv(int(uplo), n, nrhs, _p(A), max(n, 1), _p(B), max(n, 1), _p(X), max(n, 1),
+                           ctypes.addressof(it))
+    return A, X, it.value, info
+
+
+def potrs(uplo, F, B):
+    """Solve with a Cholesky factor held in the uplo triangle of F (FP64)."""
+    F, B = _f(F, np.float64), _f(B, np.float64)
+    n, nrhs = B.shape
+    lib().or_dpotrs(int(uplo), n, nrhs, _p(F), max(n, 1), _p(B), max(n, 1))
+    return B
 
 
 def dmmch_ratio(transa, transb, alpha, A, B, beta, C_in, C_out, eps=None):

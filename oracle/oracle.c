@@ -326,6 +326,102 @@ done:
   return 0;
 }
 
+/* ---- DSPOSV: lapack-netlib/SRC/dsposv.f:197-440 restated: the dsgesv loop
+ * above with the Cholesky pieces: ANRM = DLANSY('I') of the symmetric matrix
+ * held in the uplo triangle, DLAT2S (demote the triangle, overflow -> ITER =
+ * -2), SPOTRF (here the unblocked or_spotf2; info -> ITER = -3), SPOTRS, the
+ * residual with DSYMM (double accumulation over the symmetric matrix), and
+ * the FP64 DPOTRF/DPOTRS fallback on the original A (uplo 1 lower / 0 upper). */
+#define SYM(A, i, j, lda, lower) ((((lower) && (i) >= (j)) || (!(lower) && (i) <= (j))) ? (A)[IDX(i, j, lda)] : (A)[IDX(j, i, lda)])
+
+static int dlat2s(int lower, int n, const double* a, int lda, float* s, int lds) {
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      if (lower ? i < j : i > j) { s[IDX(i, j, lds)] = 0.f; continue; }
+      double v = a[IDX(i, j, lda)];
+      if (v < -(double)FLT_MAX || v > (double)FLT_MAX) return 1;
+      s[IDX(i, j, lds)] = (float)v;
+    }
+  return 0;
+}
+
+/* potrs: L L^T (lower) or U^T U (upper) in the stored triangle */
+#define POTRS_BODY(T)                                                              \
+  for (int c = 0; c < nrhs; c++) {                                                 \
+    T* x = B + IDX(0, c, ldb);                                                     \
+    for (int i = 0; i < n; i++) { /* first triangle: forward */                     \
+      T v = x[i];                                                                  \
+      for (int k = 0; k < i; k++) v -= (lower ? F[IDX(i, k, ldf)] : F[IDX(k, i, ldf)]) * x[k]; \
+      x[i] = v / F[IDX(i, i, ldf)];                                                \
+    }                                                                              \
+    for (int i = n - 1; i >= 0; i--) { /* second triangle: backward */              \
+      T v = x[i];                                                                  \
+      for (int k = i + 1; k < n; k++) v -= (lower ? F[IDX(k, i, ldf)] : F[IDX(i, k, ldf)]) * x[k]; \
+      x[i] = v / F[IDX(i, i, ldf)];                                                \
+    }                                                                              \
+  }
+void or_dpotrs(int lower, int n, int nrhs, const double* F, int ldf, double* B, int ldb) { POTRS_BODY(double) }
+void or_spotrs(int lower, int n, int nrhs, const float* F, int ldf, float* B, int ldb) { POTRS_BODY(float) }
+
+static void sym_residual(int lower, int n, int nrhs, const double* A, int lda, const double* X, int ldx,
+                         const double* B, int ldb, double* R) {
+  for (int c = 0; c < nrhs; c++)
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int l = 0; l < n; l++) s += SYM(A, i, l, lda, lower) * X[IDX(l, c, ldx)];
+      R[IDX(i, c, n)] = B[IDX(i, c, ldb)] - s;
+    }
+}
+
+int or_dsposv(int lower, int n, int nrhs, double* A, int lda, const double* B, int ldb, double* X, int ldx,
+              int* iter) {
+  *iter = 0;
+  if (n <= 0) return 0;
+  const int ITERMAX = 30;
+  double anrm = 0;
+  for (int i = 0; i < n; i++) {
+    double s = 0;
+    for (int j = 0; j < n; j++) s += fabs(SYM(A, i, j, lda, lower));
+    if (s > anrm || isnan(s)) anrm = s;
+  }
+  const double eps = ldexp(1.0, -53);
+  const double cte = anrm * eps * sqrt((double)n) * 1.0;
+  float* SA = (float*)malloc(sizeof(float) * (size_t)n * n);
+  float* SX = (float*)malloc(sizeof(float) * (size_t)n * (nrhs > 0 ? nrhs : 1));
+  double* R = (double*)malloc(sizeof(double) * (size_t)n * (nrhs > 0 ? nrhs : 1));
+  int it = 0, fb = 0;
+  if (dlag2s(n, nrhs, B, ldb, SX, n)) { it = -2; fb = 1; }
+  if (!fb && dlat2s(lower, n, A, lda, SA, n)) { it = -2; fb = 1; }
+  if (!fb && or_spotf2(lower, n, SA, n) != 0) { it = -3; fb = 1; }
+  if (!fb) {
+    or_spotrs(lower, n, nrhs, SA, n, SX, n);
+    for (int c = 0; c < nrhs; c++)
+      for (int i = 0; i < n; i++) X[IDX(i, c, ldx)] = SX[IDX(i, c, n)];
+    sym_residual(lower, n, nrhs, A, lda, X, ldx, B, ldb, R);
+    if (converged(n, nrhs, X, ldx, R, cte)) { *iter = 0; goto done; }
+    for (int k = 1; k <= ITERMAX; k++) {
+      if (dlag2s(n, nrhs, R, n, SX, n)) { it = -2; fb = 1; break; }
+      or_spotrs(lower, n, nrhs, SA, n, SX, n);
+      for (int c = 0; c < nrhs; c++)
+        for (int i = 0; i < n; i++) X[IDX(i, c, ldx)] += (double)SX[IDX(i, c, n)];
+      sym_residual(lower, n, nrhs, A, lda, X, ldx, B, ldb, R);
+      if (converged(n, nrhs, X, ldx, R, cte)) { *iter = k; goto done; }
+    }
+    if (!fb) { it = -ITERMAX - 1; fb = 1; }
+  }
+  {
+    *iter = it;
+    int info = or_dpotf2(lower, n, A, lda);
+    if (info) { free(SA); free(SX); free(R); return info; }
+    for (int c = 0; c < nrhs; c++)
+      for (int i = 0; i < n; i++) X[IDX(i, c, ldx)] = B[IDX(i, c, ldb)];
+    or_dpotrs(lower, n, nrhs, A, lda, X, ldx);
+  }
+done:
+  free(SA); free(SX); free(R);
+  return 0;
+}
+
 /* ---- ctest checker: ctest/c_dblat3c.c:2956-3127 (dmmch).  Returns the
  * largest error ratio |ct - cc| / eps / g with the gauge
  * g = |alpha| sum |a||b| + |beta| |c| (ratio not scaled where g == 0). */

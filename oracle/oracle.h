@@ -21,6 +21,10 @@ int or_dpotf2(int uplo, int n, double* A, int lda);
 int or_spotf2(int uplo, int n, float* A, int lda);
 int or_dsgesv(int n, int nrhs, double* A, int lda, int* ipiv, const double* B, int ldb, double* X, int ldx,
               int* iter);
+int or_dsposv(int lower, int n, int nrhs, double* A, int lda, const double* B, int ldb, double* X, int ldx,
+              int* iter);
+void or_dpotrs(int lower, int n, int nrhs, const double* F, int ldf, double* B, int ldb);
+void or_spotrs(int lower, int n, int nrhs, const float* F, int ldf, float* B, int ldb);
 double or_dmmch_ratio(int transa, int transb, int m, int n, int k, double alpha, const double* A, int lda,
                       const double* B, int ldb, double beta, const double* C_in, int ldc, const double* C_out,
                       int ldcc, double eps);

@@ -153,6 +153,31 @@ def dsgesv(A, B):
     return X, ipiv[:n].astype(np.int32), it.value, info.value
 
 
+def dsposv(uplo, A, B):
+    """uplo 1 lower / 0 upper.  Returns (X, iter, info); A is overwritten
+    (by its FP64 factor) only on the fallback."""
+    I = _it()
+    n, nrhs = B.shape
+    X = np.zeros((n, nrhs), order="F")
+    work = np.zeros(max(n * nrhs, 1))
+    swork = np.zeros(max(n * (n + nrhs), 1), np.float32)
+    it, info = I(0), I(0)
+    _fn("dsposv_")(c_char_p(b"L" if uplo else b"U"), byref(I(n)), byref(I(nrhs)), _p(A), byref(I(max(n, 1))), _p(B),
+                   byref(I(max(n, 1))), _p(X), byref(I(max(n, 1))), _p(work), _p(swork), byref(it), byref(info))
+    return X, it.value, info.value
+
+
+def lapacke_potrf(layout, uplo, A):
+    """LAPACKE_dpotrf(layout, uplo, n, a, lda) on the memory of A (a
+    Fortran-ordered array; for row-major the caller passes the transpose).
+    Returns the LAPACKE info."""
+    I = _it()
+    n = A.shape[0]
+    f = _fn("LAPACKE_dpotrf")
+    f.restype = I
+    return int(f(c_int(layout), ctypes.c_char(uplo), I(n), _p(A), I(max(n, 1))))
+
+
 def dgemm_batch(As, Bs, Cs, alpha=1.0, beta=0.0):
     """cblas_dgemm_batch with one group per problem (NN, column-major).
     The installed 0.3.28 build crashes (SIGSEGV) when a problem has

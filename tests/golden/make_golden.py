@@ -4,7 +4,8 @@ its exported entry points by oracle/refblas.py) for the functions on the
 level-3 hot path.  The oracle tests check the C oracle against these, the
 GPU tests check libopenblas_amd against them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py           # level3_golden.npz
+    python tests/golden/make_golden.py --extra   # level3_golden_extra.npz
 """
 import os
 import sys
@@ -130,5 +131,78 @@ def main():
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 
 
+def spd_matrix(n, kappa, rng, nref=8):
+    """Q diag(s) Q^T with Q a product of nref Householder reflectors and s
+    geometric from 1 to 1/kappa: symmetric positive definite, cond = kappa."""
+    s = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = np.diag(s)
+    for _ in range(nref):
+        v = rng.standard_normal(n)
+        w = (2.0 / (v @ v)) * v
+        A = A - np.outer(v, w @ A)          # H A
+        A = A - np.outer(A @ v, w)          # (H A) H
+    A = 0.5 * (A + A.T)
+    return np.asfortranarray(A)
+
+
+def main_extra():
+    """tests/golden/level3_golden_extra.npz: dsposv (both triangles, kappa in
+    {1e2, 1e6, 1e10}), getrf with a nonzero pivot below the smallest normal
+    number (getf2_k.c:103-116) and with NaN in a pivot column (IAMAX rule),
+    LAPACKE_dpotrf's NaN pre-check on the referenced triangle."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    for j, kappa in enumerate((1e2, 1e6, 1e10)):
+        A = spd_matrix(64, kappa, rng)
+        for uplo in (0, 1):
+            for nrhs in (1, 2):
+                B = np.asfortranarray(rng.standard_normal((64, nrhs)))
+                Aw = A.copy(order="F")
+                X, it, info = R.dsposv(uplo, Aw, B.copy(order="F"))
+                key = f"dsposv_{j}_{uplo}_{nrhs}"
+                out[key + "_A"], out[key + "_B"], out[key + "_X"], out[key + "_Aout"] = A, B, X, Aw
+                out[key + "_iter"], out[key + "_info"] = np.array([it]), np.array([info])
+    # getrf: column 3's largest remaining entry is subnormal (nonzero, < DBL_MIN)
+    for n in (8, 40):
+        M = rng.standard_normal((n, n))
+        M[:, 3] = 0.0
+        M[6, 3] = 1e-310
+        M[5, 3] = -3e-311
+        LU = np.asfortranarray(M.copy())
+        ipiv, info = R.getrf(LU)
+        out[f"getrf_tiny_{n}_A"], out[f"getrf_tiny_{n}_LU"] = M, LU
+        out[f"getrf_tiny_{n}_ipiv"], out[f"getrf_tiny_{n}_info"] = ipiv, np.array([info])
+    # getrf: NaN entries in pivot columns, one at the diagonal position, one below it
+    for t, (r, c) in enumerate(((2, 2), (7, 4))):
+        M = rng.standard_normal((12, 12))
+        M[r, c] = np.nan
+        LU = np.asfortranarray(M.copy())
+        ipiv, info = R.getrf(LU)
+        out[f"getrf_nan_{t}_A"], out[f"getrf_nan_{t}_LU"] = M, LU
+        out[f"getrf_nan_{t}_ipiv"], out[f"getrf_nan_{t}_info"] = ipiv, np.array([info])
+    # LAPACKE_dpotrf NaN pre-check: NaN in the referenced / the other triangle
+    n = 6
+    Bm = rng.standard_normal((n, n))
+    S = Bm @ Bm.T + n * np.eye(n)
+    for layout in (101, 102):
+        for uplo in (b"L", b"U"):
+            for where in ("ref", "other"):
+                A = S.copy()
+                lower_ref = uplo == b"L"   # the referenced triangle of the logical matrix, in either layout
+                i, j = (4, 1) if (where == "ref") == lower_ref else (1, 4)
+                A[i, j] = np.nan
+                buf = np.array(A if layout == 102 else A.T, order="F")   # memory: column-major view
+                info = R.lapacke_potrf(layout, uplo, buf)
+                key = f"lapacke_potrf_{layout}_{uplo.decode()}_{where}"
+                out[key + "_A"], out[key + "_info"] = A, np.array([info])
+    out["reference_config"] = np.array(R.config())
+    path = os.path.join(HERE, "level3_golden_extra.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
+
+
 if __name__ == "__main__":
-    main()
+    if "--extra" in sys.argv:
+        main_extra()
+    else:
+        main()

@@ -28,6 +28,7 @@ def L():
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"^#.*$", "", txt, flags=re.M)   # preprocessor lines (macros are not symbols)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
     skip = {"if", "defined", "sizeof"}
     return sorted({n for n in names if n not in skip and not n.startswith("CBLAS_")})

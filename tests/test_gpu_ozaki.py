@@ -81,7 +81,8 @@ def test_ozaki_edges_and_errors(oba):
 @pytest.mark.parametrize("phi", [0.5, 2.0, 8.0])
 def test_ozaki_8192_phi_inputs(oba, phi):
     """configs: M=N=K=8192, entries u*exp(phi*g); error vs native FP64 DGEMM
-    on 32 sampled rows, normwise relative to |A||B|."""
+    on 32 sampled rows, normwise relative to |A||B|: FP64 level for every
+    phi (phi = 8 through the native-GEMM fallback)."""
     L = oba.lib()
     N = 8192
     rng = np.random.default_rng(int(phi * 10))
@@ -95,8 +96,12 @@ def test_ozaki_8192_phi_inputs(oba, phi):
     gauge = np.abs(A[rows]) @ np.abs(B)
     rel = float(np.max(np.abs(out[rows] - ref) / gauge))
     print(f"phi={phi} slices={ns} max |C-Cref|/(|A||B|) = {rel:.3e}")
-    assert 2 <= ns <= 12
     if phi <= 2.0:
-        assert rel < 1e-13
+        assert 2 <= ns <= 12
+    else:
+        # the spread needs more than 12 slices: the native FP64 GEMM runs and
+        # the return value reports the required count
+        assert ns <= -13
+    assert rel < 1e-13
     for d in (dA, dB, dC):
         d.free()
