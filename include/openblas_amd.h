@@ -223,6 +223,15 @@ int openblas_amd_device_info(char *arch, int len, int *num_cu, int *clock_khz);
 /* HIP events on the library's stream: brackets exactly the library's kernels */
 void openblas_amd_timer_start(void);
 float openblas_amd_timer_stop(void);
+/* kernel timer: when enabled, the library brackets its main kernels (LU
+   panels and trailing updates, look-back solves, the GEMM/TRSM entry
+   points, the INT8-slice product, the batched kernels) with HIP events on
+   the stream each runs on and records their algorithmic flops and HBM
+   bytes; read returns one named aggregate (launches, total ms, flops,
+   bytes; 0, or -1 if nothing was recorded under that name) */
+void openblas_amd_ktimer_enable(int on);
+int openblas_amd_ktimer_read(const char *name, long *count, double *ms, double *flops, double *bytes);
+void openblas_amd_ktimer_reset(void);
 /* last argument error reported through xerbla on this thread (0 if none);
    copies the routine name; clears the record */
 int openblas_amd_last_xerbla(char *name, int len);

@@ -106,6 +106,10 @@ _SIGS = {
     "openblas_amd_timer_stop": (c_float, []),
     "openblas_amd_last_xerbla": (c_int, [c_char_p, c_int]),
     "openblas_amd_last_error": (c_int, [c_char_p, c_int]),
+    "openblas_amd_ktimer_enable": (None, [c_int]),
+    "openblas_amd_ktimer_read": (c_int, [c_char_p, POINTER(ctypes.c_long), POINTER(c_double), POINTER(c_double),
+                                         POINTER(c_double)]),
+    "openblas_amd_ktimer_reset": (None, []),
 }
 
 _lib = None

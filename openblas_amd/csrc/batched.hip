@@ -1153,6 +1153,12 @@ void gemm_batched(int ta, int tb, std::vector<BDesc>& probs) {
   Cfg c = tune(signature(BK_GEMM, sizeof(T), ta | (tb << 1), probs), cands, launch_gemm<T>, ctx, probs, wl, nw, 2, 0,
                1, sizeof(T));
   if (const char* e = getenv("OPENBLAS_AMD_BATCH_GEMM_TILE")) c.var = atoi(e) == 64 ? 1 : 0;
+  double fl = 0, by = 0;
+  for (auto& d : probs) {
+    fl += 2.0 * d.m * d.n * d.k;
+    by += sizeof(T) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta != 0.0 ? 2.0 : 1.0) * d.m * d.n);
+  }
+  KTimer kt("batched_gemm", rt().stream, fl, by);
   run_plan(tslot<T>(BK_GEMM), launch_gemm<T>, c, ctx, sizeof ctx, probs, wl, nw);
 }
 
@@ -1171,7 +1177,17 @@ void getrf_batched(std::vector<BDesc>& probs, int* info_host) {
     Cfg c = tune(signature(BK_GETRF, sizeof(T), ctx[0], small), cands, launch_getrf<T>, ctx, small, none, 0, 0, 0, 1,
                  sizeof(T));
     if (const char* e = getenv("OPENBLAS_AMD_BATCH_LU_W")) c.var = atoi(e) == 32 ? 1 : 0;
-    const int* di = run_plan(tslot<T>(BK_GETRF), launch_getrf<T>, c, ctx, sizeof ctx, small, none, 0);
+    double fl = 0, by = 0;
+    for (auto& d : small) {
+      const double mm = d.m, nn = d.n, k = std::min(mm, nn);
+      fl += mm * nn * k - (mm + nn) * k * k / 2.0 + k * k * k / 3.0;   // LU of m x n (LAPACK count, leading terms)
+      by += 2.0 * sizeof(T) * mm * nn;
+    }
+    const int* di;
+    {
+      KTimer kt("batched_getrf", rt().stream, fl, by);
+      di = run_plan(tslot<T>(BK_GETRF), launch_getrf<T>, c, ctx, sizeof ctx, small, none, 0);
+    }
     collect_info(di, small, info_host);
   }
   for (auto& d : big)
@@ -1192,7 +1208,16 @@ void potrf_batched(bool lower, std::vector<BDesc>& probs, int* info_host) {
     const std::vector<char> none;
     Cfg c = tune(signature(BK_POTRF, sizeof(T), ctx[0] * 2 + ctx[1], small), cands, launch_potrf<T>, ctx, small, none,
                  0, 0, 1, 1, sizeof(T));
-    const int* di = run_plan(tslot<T>(BK_POTRF), launch_potrf<T>, c, ctx, sizeof ctx, small, none, 0);
+    double fl = 0, by = 0;
+    for (auto& d : small) {
+      fl += (double)d.n * d.n * d.n / 3.0;
+      by += sizeof(T) * (double)d.n * (d.n + 1);   // the triangle read and written
+    }
+    const int* di;
+    {
+      KTimer kt("batched_potrf", rt().stream, fl, by);
+      di = run_plan(tslot<T>(BK_POTRF), launch_potrf<T>, c, ctx, sizeof ctx, small, none, 0);
+    }
     collect_info(di, small, info_host);
   }
   for (auto& d : big) info_host[d.idx] = potrf<T>(lower, d.n, (T*)d.a, d.lda, rt().stream);
@@ -1213,6 +1238,12 @@ void getrs_batched(bool trans, std::vector<BDesc>& probs, int* info_host) {
     const std::vector<char> none;
     Cfg c = tune(signature(BK_GETRS, sizeof(T), ctx[0] * 2 + ctx[1], small), cands, launch_getrs<T>, ctx, small, none,
                  0, 1, 1, 2, sizeof(T));
+    double fl = 0, by = 0;
+    for (auto& d : small) {
+      fl += 2.0 * d.n * d.n * std::max(d.k, 0);
+      by += sizeof(T) * ((double)d.n * d.n + 2.0 * d.n * std::max(d.k, 0));
+    }
+    KTimer kt("batched_getrs", rt().stream, fl, by);
     run_plan(tslot<T>(BK_GETRS), launch_getrs<T>, c, ctx, sizeof ctx, small, none, 0);
   }
   for (auto& d : big) {   // large problems: the single-problem solve

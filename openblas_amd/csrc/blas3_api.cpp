@@ -46,7 +46,11 @@ void gemm_run(int transa, int transb, long m, long n, long k, T alpha, const T* 
     B.in(const_cast<T*>(b), transb ? n : k, transb ? k : n, ldb, true);
   }
   C.in(c, m, n, ldc, beta != T(0));
-  gemm<T>(transa, transb, m, n, k, alpha, A.d, A.ld, B.d, B.ld, beta, C.d, C.ld, r.stream);
+  {
+    KTimer kt(sizeof(T) == 8 ? "dgemm" : "sgemm", r.stream, only_scale ? 0.0 : 2.0 * m * n * k,
+              (double)sizeof(T) * ((only_scale ? 0.0 : (double)m * k + (double)k * n) + (beta != T(0) ? 2.0 : 1.0) * m * n));
+    gemm<T>(transa, transb, m, n, k, alpha, A.d, A.ld, B.d, B.ld, beta, C.d, C.ld, r.stream);
+  }
   C.out();
   end_call();
 }
@@ -118,6 +122,8 @@ void trsm_run(int side, int uplo, int trans, int unit, long m, long n, T alpha, 
   } else {
     A.in(const_cast<T*>(a), na, na, lda, true);
     // side: 0 left / 1 right; uplo: 0 upper / 1 lower; unit: 0 unit / 1 non-unit
+    KTimer kt(sizeof(T) == 8 ? "dtrsm" : "strsm", r.stream, (double)m * n * (double)na,
+              (double)sizeof(T) * (0.5 * na * na + 2.0 * m * n));
     trsm<T>(side == 1, uplo == 1, trans == 1, unit == 0, m, n, alpha, A.d, A.ld, B.d, B.ld, r.stream);
   }
   B.out();

@@ -114,6 +114,22 @@ struct Mat {
   void out();                                              // copy back
 };
 
+// Opt-in kernel timer (openblas_amd_ktimer_*): when enabled, named launches
+// are bracketed by HIP events on the stream they run on, with the launch's
+// algorithmic flops and HBM bytes; reading resolves the events.  Off by
+// default: then a KTimer costs one branch.
+bool ktimer_on();
+struct KTimer {
+  const char* name = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr;
+  double flops = 0, bytes = 0;
+  KTimer(const char* n, hipStream_t st, double fl, double by);
+  ~KTimer();
+  KTimer(const KTimer&) = delete;
+  KTimer& operator=(const KTimer&) = delete;
+};
+
 void end_call();   // synchronize if the runtime is in synchronous mode; check_device_error
 // throws DeviceError if a kernel has set the device error word (call after a
 // synchronization that covers the kernels in question)

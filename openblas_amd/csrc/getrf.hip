@@ -504,6 +504,440 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   }
 }
 
+// ---- panel kernel, second form ---------------------------------------------------
+// Same contract as lu_panel_kernel, shaped for fewer instructions per column:
+//  * 256-thread workgroups (one wavefront per SIMD) with RPT rows per thread,
+//    so the per-column fixed work (reductions, barriers, the pivot-row
+//    reads) is paid by 4 wavefronts instead of 8;
+//  * pivot keys in the panel's own precision (NaN mapped so that it wins only
+//    at the current position, kernel/arm/iamax.c:51-74): a wavefront max of
+//    the key, then the lowest position among the lanes holding that key, each
+//    a DPP butterfly within the 16-lane rows plus four readlanes;
+//  * rows rotated by U columns once every U columns (column j0+u of a group
+//    sits in register u of the group's rotation), so each of the U unrolled
+//    column bodies has static register indices and the rotation costs W/U
+//    moves per column instead of W;
+//  * each wavefront keeps only the best record of its share in LDS, so the
+//    LDS footprint does not grow with the workgroup count (P is bounded by
+//    the record area and by residency, not by LDS).
+// With P > 1 the tail of column j's row update is deferred into column j+1's
+// exchange wait (as in lu_panel_kernel).
+constexpr int L2_NT = 256;
+constexpr int L2_NW = L2_NT / 64;
+constexpr int L2_U = 4;           // columns per rotation group
+constexpr int L2_PMAX = 1024;     // record slots per column buffer
+constexpr int L2_EB = 8;          // records a wavefront polls at once
+
+template <typename T>
+__device__ inline T lu2_key(T x, int pos, int j) {
+  return isnan(x) ? (pos == j ? T(INFINITY) : T(-1)) : fabs(x);
+}
+
+template <int CTRL>
+__device__ inline float lu2_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double lu2_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ inline int lu2_dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ inline float lu2_rl(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double lu2_rl(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// (key, position) best of the wavefront, uniform in every lane: the largest
+// key, ties to the lowest position (IAMAX: first maximum)
+template <typename T>
+__device__ __attribute__((always_inline)) inline void lu2_wave_best(T& key, int& pos) {
+  T m = key;
+  m = fmax(m, lu2_dpp<0xB1>(m));
+  m = fmax(m, lu2_dpp<0x4E>(m));
+  m = fmax(m, lu2_dpp<0x141>(m));
+  m = fmax(m, lu2_dpp<0x140>(m));
+  m = fmax(fmax(lu2_rl(m, 0), lu2_rl(m, 16)), fmax(lu2_rl(m, 32), lu2_rl(m, 48)));
+  int p = key == m ? pos : INT_MAX;
+  p = min(p, lu2_dpp<0xB1>(p));
+  p = min(p, lu2_dpp<0x4E>(p));
+  p = min(p, lu2_dpp<0x141>(p));
+  p = min(p, lu2_dpp<0x140>(p));
+  p = min(min(__builtin_amdgcn_readlane(p, 0), __builtin_amdgcn_readlane(p, 16)),
+          min(__builtin_amdgcn_readlane(p, 32), __builtin_amdgcn_readlane(p, 48)));
+  key = m;
+  pos = p;
+}
+
+// rotate a register row left by U elements (static indices only)
+template <int U, typename T, int W>
+__device__ __attribute__((always_inline)) inline void lu2_rotate(T (&r)[W]) {
+  T t[U];
+#pragma unroll
+  for (int k = 0; k < U; k++) t[k] = r[k];
+#pragma unroll
+  for (int k = 0; k < W - U; k++) r[k] = r[k + U];
+#pragma unroll
+  for (int k = 0; k < U; k++) r[W - U + k] = t[k];
+}
+
+// x[k] -= l * u[k + OFF] for k in [K0, W-1] with k <= kmax (kmax uniform):
+// chunks of 4 under one uniform branch, selects only in the straddling chunk
+template <int K0, int OFF, typename T, int W>
+__device__ __attribute__((always_inline)) inline void lu2_tail(T (&x)[W], T l, const T* u, int kmax) {
+#pragma unroll
+  for (int c0 = K0; c0 < W; c0 += 4) {
+    if (c0 > kmax) break;
+    T uc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) uc[k] = (c0 + k < W && c0 + k + OFF < W) ? u[c0 + k + OFF] : T(0);
+    if (c0 + 3 <= kmax) {
+#pragma unroll
+      for (int k = c0; k < c0 + 4; k++)
+        if (k < W) x[k] = lu_fms(l, uc[k - c0], x[k]);
+    } else {
+#pragma unroll
+      for (int k = c0; k < c0 + 4; k++) {
+        if (k < W) {
+          const T f = lu_fms(l, uc[k - c0], x[k]);
+          x[k] = k <= kmax ? f : x[k];
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int W, int RPT>
+__global__ __launch_bounds__(L2_NT) void lu_panel2_kernel(T* __restrict__ At, long ldt, long m, long j0, int w,
+                                                          int* __restrict__ ipiv, int* __restrict__ info,
+                                                          int* __restrict__ moves, unsigned long long* __restrict__ rec,
+                                                          unsigned base, unsigned* err, int pre) {
+  constexpr int NT = L2_NT, NW = L2_NW, U = L2_U;
+  constexpr int ROWS = NT * RPT;
+  static_assert(W % U == 0 && W <= 64, "rotation groups must tile the panel");
+  const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const long mp = m - j0;
+  constexpr int DW = W * (int)sizeof(T) / 4;   // 32-bit words of a row
+  constexpr int RW = DW + 1;                   // + the position word
+  constexpr int RPI = 64 / DW;                 // records per load instruction
+  static_assert(DW <= 64 && 64 % DW == 0, "a load instruction covers whole records");
+  __shared__ T s_cand[2][NW][W];     // each wavefront's best local row
+  __shared__ T s_key[2][NW];
+  __shared__ int s_pos[2][NW];
+  __shared__ T s_best[2][NW][W];     // each wavefront's best polled record row (P > 1)
+  __shared__ T s_bkey[2][NW];
+  __shared__ int s_bpos[2][NW];
+  __shared__ int s_bok[2][NW];
+  __shared__ int s_ipiv[W];
+  __shared__ T s_u12[W][W];
+  __shared__ T s_stage[W * 2 * W];   // L11 | A12 staging for `pre`
+  int zinfo = 0;
+
+  T x[RPT][W];
+  int pos[RPT];
+  bool live[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const long r = (long)g * ROWS + q * NT + t;
+    live[q] = r < mp;
+    pos[q] = (int)r;
+    const T* src = At + j0 + (j0 + r) * ldt;
+#pragma unroll
+    for (int c = 0; c < W; c++) x[q][c] = (live[q] && c < w) ? src[c] : T(0);
+  }
+  // pre: the node's U12 = L11^{-1} A12 and each row's x -= l(row) U12 (see
+  // lu_panel_kernel); the k loop stays rolled to keep the code small
+  if (pre) {
+    for (int idx = t; idx < W * 2 * W; idx += NT) {
+      const int k = idx / (2 * W), e = idx % (2 * W);
+      s_stage[idx] = (e < W || e - W < w) ? At[(j0 - W + e) + (j0 - W + k) * ldt] : T(0);
+    }
+    __syncthreads();
+    if (t < W) {
+      T u[W];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        T v = s_stage[k * 2 * W + W + t];
+#pragma unroll
+        for (int kk = 0; kk < k; kk++) v = lu_fms(s_stage[k * 2 * W + kk], u[kk], v);
+        u[k] = v;
+      }
+#pragma unroll
+      for (int k = 0; k < W; k++) s_u12[k][t] = u[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      if (live[q]) {
+        const long r = (long)g * ROWS + q * NT + t;
+        const T* lr = At + (j0 - W) + (j0 + r) * ldt;
+#pragma unroll 1
+        for (int k = 0; k < W; k++) {
+          const T l = lr[k];
+#pragma unroll
+          for (int c = 0; c < W; c++) x[q][c] = lu_fms(l, s_u12[k][c], x[q][c]);
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0x0F70);   // the row loads complete here, once (vmcnt(0))
+
+  T lqp[RPT];
+  bool updp[RPT];
+  int frp = 0;   // the previous column's pivot record slot in s_best[(j-1)&1] (P > 1)
+#pragma unroll
+  for (int q = 0; q < RPT; q++) { lqp[q] = T(0); updp[q] = false; }
+  bool failed = false;
+  int jg = 0;   // first column of the current rotation group
+  // one column of the current group; u (static) is the column's register
+  auto column = [&](auto u_tag) __attribute__((always_inline)) {
+      constexpr int u = decltype(u_tag)::value;
+      const int j = jg + u;
+      if (j >= w || failed) return;
+      const int buf = j & 1;
+      const int kmax = W - 1 - j;    // elements right of column j: x[u+1 .. u+kmax]
+      // (a) thread best over its rows, then the wavefront's
+      T bk = T(-2);
+      int bp = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        if (live[q]) {
+          const T kq = lu2_key(x[q][u], pos[q], j);
+          if (kq > bk || (kq == bk && pos[q] < bp)) { bk = kq; bp = pos[q]; }
+        }
+      }
+      lu2_wave_best(bk, bp);
+#pragma unroll
+      for (int q = 0; q < RPT; q++)
+        if (live[q] && pos[q] == bp) {
+#pragma unroll
+          for (int c = 0; c < W; c++) s_cand[buf][wave][c] = x[q][c];
+        }
+      if (lane == 0) { s_key[buf][wave] = bk; s_pos[buf][wave] = bp; }
+      __syncthreads();
+      // (b) the workgroup's best (every thread reduces the same values)
+      T wk = s_key[buf][0];
+      int wp = s_pos[buf][0], wv = 0;
+#pragma unroll
+      for (int v = 1; v < NW; v++) {
+        const T kv = s_key[buf][v];
+        const int pv = s_pos[buf][v];
+        if (kv > wk || (kv == wk && pv < wp)) { wk = kv; wp = pv; wv = v; }
+      }
+      int gp;
+      const T* urow;
+      // the previous column's pivot row (group rotation of column j-1: for
+      // u == 0 it was staged in the previous group's rotation, U further)
+      constexpr int POFF = (u == 0) ? U : 0;
+      if (P == 1) {
+        gp = wp;
+        urow = &s_cand[buf][wv][0];
+      } else {
+        const unsigned tag = base + (unsigned)(j + 1);
+        const unsigned long long tg = (unsigned long long)tag << 32;
+        unsigned long long* recs = rec + (size_t)buf * L2_PMAX * RW;
+        const T* uprev = &s_best[buf ^ 1][frp][0];
+        // (c) publish the workgroup's candidate, with column j-1's deferred
+        // tail applied (its multiplier is element u-1, or W-1 when u == 0)
+        if (wave == 0) {
+          unsigned long long* dst = recs + (size_t)g * RW;
+          constexpr int EPL = (int)sizeof(T) / 4;   // words per element
+          const int e = lane / EPL;                 // element of this lane's word
+          if (lane < DW) {
+            T v = s_cand[buf][wv][e];
+            const T lprev = s_cand[buf][wv][(u + W - 1) % W];
+            if (j > 0 && e >= u + 1 && e <= u + kmax) v = lu_fms(lprev, uprev[e + POFF < W ? e + POFF : W - 1], v);
+            unsigned word;
+            if constexpr (sizeof(T) == 8) word = (lane & 1) ? (unsigned)__double2hiint(v) : (unsigned)__double2loint(v);
+            else word = (unsigned)__float_as_int(v);
+            wt_store(dst + lane, tg | word);
+          }
+          if (lane == 0) wt_store(dst + DW, tg | (unsigned)wp);
+        }
+        // (d) the deferred tail of column j-1 on this thread's rows
+        if (j > 0) {
+#pragma unroll
+          for (int q = 0; q < RPT; q++)
+            if (updp[q]) lu2_tail<u + 1, POFF, T, W>(x[q], lqp[q], uprev, u + kmax);
+        }
+        // (e) wavefront v polls records v*RPI.., (v+NW)*RPI.., ...; lane
+        // h*DW + d holds word d of record (group base + h); the best record
+        // of the share goes to s_best[buf][v]
+        constexpr int GS = NW * RPI;
+        const int h = lane / DW, d = lane % DW;
+        T vbk = T(-2);
+        int vbp = INT_MAX;
+        bool ok = true;
+        for (int c0 = wave * RPI; c0 < P && ok; c0 += L2_EB * GS) {
+          unsigned long long wd[L2_EB], wq[L2_EB];
+          bool have[L2_EB];
+#pragma unroll
+          for (int k = 0; k < L2_EB; k++) {
+            have[k] = c0 + k * GS + h >= P;
+            wd[k] = 0;
+            wq[k] = 0;
+          }
+          long spins = 0;
+          while (true) {
+#pragma unroll
+            for (int k = 0; k < L2_EB; k++) {
+              if (!have[k]) {
+                const unsigned long long* src = recs + (size_t)(c0 + k * GS + h) * RW;
+                wd[k] = bypass_load(src + d);
+                if (d == 0) wq[k] = bypass_load(src + DW);
+              }
+            }
+            bool all = true;
+#pragma unroll
+            for (int k = 0; k < L2_EB; k++) {
+              if (!have[k]) have[k] = (unsigned)(wd[k] >> 32) == tag && (d != 0 || (unsigned)(wq[k] >> 32) == tag);
+              all = all && have[k];
+            }
+            if (__all(all)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1L << 22)) { ok = false; break; }
+          }
+          if (!ok) break;
+#pragma unroll
+          for (int k = 0; k < L2_EB; k++) {
+            const int rb = c0 + k * GS;
+            if (rb >= P) break;
+            const int wdi = (int)(unsigned)wd[k], wqi = (int)(unsigned)wq[k];
+#pragma unroll
+            for (int hh = 0; hh < RPI; hh++) {
+              if (rb + hh < P) {
+                const int cp = __builtin_amdgcn_readlane(wqi, hh * DW);
+                T key;   // element u of the record: column j
+                if constexpr (sizeof(T) == 8) {
+                  const int lo = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * u);
+                  const int hi = __builtin_amdgcn_readlane(wdi, hh * DW + 2 * u + 1);
+                  key = __hiloint2double(hi, lo);
+                } else {
+                  key = __int_as_float(__builtin_amdgcn_readlane(wdi, hh * DW + u));
+                }
+                if (cp != INT_MAX) {
+                  const T kv = lu2_key(key, cp, j);
+                  if (kv > vbk || (kv == vbk && cp < vbp)) {
+                    vbk = kv;
+                    vbp = cp;
+                    // stage this record's row (the lanes that hold it)
+                    if (h == hh) reinterpret_cast<unsigned*>(&s_best[buf][wave][0])[d] = (unsigned)wd[k];
+                  }
+                }
+              }
+            }
+          }
+        }
+        if (lane == 0) { s_bkey[buf][wave] = vbk; s_bpos[buf][wave] = vbp; s_bok[buf][wave] = ok ? 1 : 0; }
+        __syncthreads();
+        // (f) the column's pivot: every thread reduces the same values
+        T fk = s_bkey[buf][0];
+        int fp = s_bpos[buf][0], fv = 0;
+        bool allok = s_bok[buf][0] != 0;
+#pragma unroll
+        for (int v = 1; v < NW; v++) {
+          allok = allok && s_bok[buf][v] != 0;
+          const T kv = s_bkey[buf][v];
+          const int pv = s_bpos[buf][v];
+          if (kv > fk || (kv == fk && pv < fp)) { fk = kv; fp = pv; fv = v; }
+        }
+        if (!allok || fp < j || fp >= mp) {   // a wait ran out; the rest cannot happen
+          if (t == 0) atomicExch(err, allok ? 2u : 1u);
+          failed = true;
+          return;
+        }
+        gp = fp;
+        urow = &s_best[buf][fv][0];
+        frp = fv;
+      }
+      const T piv = urow[u];
+      const bool nz = piv != T(0);
+      const bool doscale = nz && fabs(piv) >= MinNormal<T>::v;
+      if (t == 0) s_ipiv[j] = (int)(j0 + gp + 1);
+      if (!nz && zinfo == 0) zinfo = (int)(j0 + j + 1);
+      const T rinv = doscale ? T(1) / piv : T(1);
+      const bool defer = P > 1;
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        bool upd = false;
+        T lq = T(0);
+        if (live[q]) {
+          if (pos[q] == gp) {   // the pivot row: now a row of U
+            pos[q] = j;
+            live[q] = false;
+          } else {
+            if (pos[q] == j) pos[q] = gp;
+            lq = doscale ? x[q][u] * rinv : x[q][u];
+            upd = true;
+          }
+        }
+        if (upd) {
+          x[q][u] = lq;
+          if (kmax >= 1) x[q][u + 1] = lu_fms(lq, urow[u + 1], x[q][u + 1]);
+          if (!defer) lu2_tail<u + 2, 0, T, W>(x[q], lq, urow, u + kmax);
+        }
+        lqp[q] = lq;
+        updp[q] = upd;
+      }
+  };
+  static_assert(U == 4, "the group below calls four column bodies");
+#pragma unroll 1
+  for (; jg < w && !failed; jg += U) {
+    column(std::integral_constant<int, 0>{});
+    column(std::integral_constant<int, 1>{});
+    column(std::integral_constant<int, 2>{});
+    column(std::integral_constant<int, 3>{});
+    if (failed) break;
+    // next group: rotate by U (column jg+U moves to register 0)
+    if (jg + U < w) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) lu2_rotate<U, T, W>(x[q]);
+    }
+  }
+  if (g == 0) {   // the pivots and the first zero pivot; U12
+    __syncthreads();
+    if (t < w && !failed) ipiv[j0 + t] = s_ipiv[t];
+    if (t == 0 && zinfo != 0 && *info == 0) *info = zinfo;
+    if (pre) {
+      for (int idx = t; idx < W * w; idx += NT) {
+        const int k = idx / w, c = idx % w;
+        At[(j0 + c) + (j0 - W + k) * ldt] = s_u12[k][c];
+      }
+    }
+  }
+  if (failed || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  // back to column order: the rotation so far is the start of the last group
+  const int rot = (jg < w ? jg : (w - 1) / U * U) % W;
+#pragma unroll 1
+  for (int k = rot; k % W != 0; k += U) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) lu2_rotate<U, T, W>(x[q]);
+  }
+  // every row to its final position; list the rows that moved
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const long r = (long)g * ROWS + q * NT + t;
+    if (r < mp) {
+      T* dst = At + j0 + (j0 + pos[q]) * ldt;
+#pragma unroll
+      for (int c = 0; c < W; c++)
+        if (c < w) dst[c] = x[q][c];
+      if (pos[q] != (int)r) {
+        const int k = atomicAdd(moves, 1);
+        if (k < 2 * W) {
+          moves[1 + 2 * k] = pos[q];
+          moves[2 + 2 * k] = (int)r;
+        }
+      }
+    }
+  }
+}
+
 // Apply a panel's interchanges to the columns of A in [lo, hi) outside
 // [ex_lo, ex_hi), i.e. to those rows of At: the moves (destination position,
 // source row, both relative to the panel's first row j0) form a permutation
@@ -620,18 +1054,64 @@ struct LuCtx {
   hipStream_t s;
 };
 
+// rows per thread of lu_panel2_kernel
+constexpr int L2_RPT = 2;
+
+// which panel kernel factors the leaves: the second form unless
+// OPENBLAS_AMD_LU_PANEL=1 asks for the first (kept for comparison)
+static bool lu_use_panel2() {
+  static const bool v = [] {
+    const char* e = getenv("OPENBLAS_AMD_LU_PANEL");
+    return !(e && atoi(e) == 1);
+  }();
+  return v;
+}
+
+// Workgroups of lu_panel2_kernel that the device can hold at once: all of a
+// launch must be resident together (they exchange candidates every column).
+template <typename T>
+static long lu_panel2_capacity() {
+  static long cap = -1;
+  if (cap < 0) {
+    int per_cu = 0;
+    OBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lu_panel2_kernel<T, LU_W, L2_RPT>, L2_NT, 0));
+    cap = (long)std::max(per_cu, 0) * rt().num_cu;
+  }
+  return cap;
+}
+
 template <typename T>
 static void lu_panel(LuCtx<T>& c, long j0, int w, bool pre = false) {
   const long mp = c.m - j0;
-  constexpr int RPT = LuRpt<T>::v;
-  constexpr int NT = LuNt<T>::v;
-  constexpr long ROWS = (long)NT * RPT;
-  long P = (mp + ROWS - 1) / ROWS;
-  if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
   int* mv = c.moves + c.panel_no * LU_MV;
-  lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
-                                                                        c.rec, c.base, c.err, pre ? 1 : 0);
+  long P;
+  // algorithmic work of the leaf: per column j, (mp-1-j) multipliers and a
+  // rank-1 update of (mp-1-j) x (w-1-j); with `pre`, the node's W x w solve
+  // and the mp x w x W update; bytes: the panel read and written once
+  // (+ the W multipliers per row with `pre`)
+  double lf = 0;
+  for (int j = 0; j < w; j++) lf += (double)(mp - 1 - j) * (2.0 * (w - 1 - j) + 1.0);
+  if (pre) lf += (double)LU_W * LU_W * w + 2.0 * mp * LU_W * w;
+  {
+  KTimer kt("lu_panel", c.s, lf, (double)sizeof(T) * mp * (2.0 * w + (pre ? LU_W : 0)));
+  if (lu_use_panel2()) {
+    constexpr long ROWS = (long)L2_NT * L2_RPT;
+    P = (mp + ROWS - 1) / ROWS;
+    if (P > L2_PMAX || P > lu_panel2_capacity<T>())
+      fatal("getrf: more rows than the panel kernel can hold resident at once", nullptr);
+    lu_panel2_kernel<T, LU_W, L2_RPT><<<(unsigned)P, L2_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
+                                                                      c.rec, c.base, c.err, pre ? 1 : 0);
+  } else {
+    constexpr int RPT = LuRpt<T>::v;
+    constexpr int NT = LuNt<T>::v;
+    constexpr long ROWS = (long)NT * RPT;
+    P = (mp + ROWS - 1) / ROWS;
+    if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
+    lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
+                                                                          c.rec, c.base, c.err, pre ? 1 : 0);
+  }
   OBA_HIP(hipGetLastError());
+  }
   if (P > 1) c.base += (unsigned)w;   // record tags of this panel: base+1 .. base+w
   c.panel_no++;
   c.leaves.push_back(LuLeaf{j0, mv});
@@ -657,9 +1137,11 @@ static void lu_rec(LuCtx<T>& c, long j0, long nc) {
   trsm<T>(true, false, false, true, n2, n1, T(1), At + j0 + j0 * ldt, ldt, At + (j0 + n1) + j0 * ldt, ldt, c.s);
   // A22 -= A21*A12   <=>   At(j0+n1:j0+nc, j0+n1:m) -= X * At(j0:j0+n1, j0+n1:m)
   const long below = c.m - (j0 + n1);
-  if (below > 0)
+  if (below > 0) {
+    KTimer kt("lu_node_gemm", c.s, 2.0 * n2 * below * n1, (double)sizeof(T) * (n2 * n1 + n1 * below + 2.0 * n2 * below));
     gemm<T>(false, false, n2, below, n1, T(-1), At + (j0 + n1) + j0 * ldt, ldt, At + j0 + (j0 + n1) * ldt, ldt,
             T(1), At + (j0 + n1) + (j0 + n1) * ldt, ldt, c.s);
+  }
   lu_rec<T>(c, j0 + n1, n2);
 }
 
@@ -711,8 +1193,11 @@ static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
     trsm_right_prepare<T>(false, false, true, w, u, ldt, inv, M);
     // next block: At rows [jn, jn+w1)
     trsm_right_prepared<T>(false, false, true, w1, w, u, ldt, At + jn + jb * ldt, ldt, inv, M);
-    gemm<T>(false, false, w1, below, w, T(-1), At + jn + jb * ldt, ldt, At + jb + jn * ldt, ldt, T(1),
-            At + jn + jn * ldt, ldt, M);
+    {
+      KTimer kt("lu_next_gemm", M, 2.0 * w1 * below * w, (double)sizeof(T) * (w1 * w + w * below + 2.0 * w1 * below));
+      gemm<T>(false, false, w1, below, w, T(-1), At + jn + jb * ldt, ldt, At + jb + jn * ldt, ldt, T(1),
+              At + jn + jn * ldt, ldt, M);
+    }
     // trailing columns [jn+w1, mn) on the bulk stream
     const long rest = mn - (jn + w1);
     if (rest > 0) {
@@ -720,8 +1205,12 @@ static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
       OBA_HIP(hipStreamWaitEvent(X, r.ev_a, 0));
       T* xr = At + (jn + w1) + jb * ldt;
       trsm_right_prepared<T>(false, false, true, rest, w, u, ldt, xr, ldt, inv, X);
-      gemm<T>(false, false, rest, below, w, T(-1), xr, ldt, At + jb + jn * ldt, ldt, T(1), At + (jn + w1) + jn * ldt,
-              ldt, X);
+      {
+        KTimer kt("lu_trailing_gemm", X, 2.0 * rest * below * w,
+                  (double)sizeof(T) * (rest * w + w * below + 2.0 * rest * below));
+        gemm<T>(false, false, rest, below, w, T(-1), xr, ldt, At + jb + jn * ldt, ldt, T(1),
+                At + (jn + w1) + jn * ldt, ldt, X);
+      }
       OBA_HIP(hipEventRecord(r.ev_b, X));
       pending = true;
     }
@@ -747,7 +1236,7 @@ int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hip
   const long mn = std::min(m, n);
   if (mn <= 0) return 0;
   const long npanels = (mn + LU_W - 1) / LU_W + 1;
-  const size_t rec_bytes = sizeof(unsigned long long) * LuRec<T, LU_W>::RW * 2 * LU_PMAX;
+  const size_t rec_bytes = sizeof(unsigned long long) * LuRec<T, LU_W>::RW * 2 * std::max(LU_PMAX, L2_PMAX);
   const size_t mv_bytes = sizeof(int) * npanels * LU_MV;
   DevBuf ws(4096 + rec_bytes + mv_bytes);
   char* p = ws.as<char>();
@@ -813,14 +1302,19 @@ __global__ void permute_rows_kernel(long n, long nrhs, const int* __restrict__ p
 }
 
 template <typename T>
+void permute_rows_ws(bool inverse, long n, long nrhs, const int* perm_dev, T* b, long ldb, T* tmp, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return;
+  const long total = n * nrhs;
+  permute_rows_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, nrhs, perm_dev, b, ldb, tmp, n, inverse);
+  OBA_HIP(hipGetLastError());
+  OBA_HIP(hipMemcpy2DAsync(b, sizeof(T) * ldb, tmp, sizeof(T) * n, sizeof(T) * n, nrhs, hipMemcpyDeviceToDevice, s));
+}
+
+template <typename T>
 void permute_rows(bool inverse, long n, long nrhs, const int* perm_dev, T* b, long ldb, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return;
   DevBuf tmp(sizeof(T) * (size_t)n * (size_t)nrhs);
-  const long total = n * nrhs;
-  permute_rows_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, nrhs, perm_dev, b, ldb, tmp.as<T>(), n,
-                                                                         inverse);
-  OBA_HIP(hipGetLastError());
-  OBA_HIP(hipMemcpy2DAsync(b, sizeof(T) * ldb, tmp.p, sizeof(T) * n, sizeof(T) * n, nrhs, hipMemcpyDeviceToDevice, s));
+  permute_rows_ws<T>(inverse, n, nrhs, perm_dev, b, ldb, tmp.as<T>(), s);
 }
 
 template <typename T>
@@ -932,6 +1426,7 @@ template int getrf_perm<double>(long, long, double*, long, int*, int*, hipStream
 template int getrf_perm<float>(long, long, float*, long, int*, int*, hipStream_t);
 template int getrf<float>(long, long, float*, long, int*, hipStream_t);
 template void permute_rows<double>(bool, long, long, const int*, double*, long, hipStream_t);
+template void permute_rows_ws<double>(bool, long, long, const int*, double*, long, double*, hipStream_t);
 template void permute_rows<float>(bool, long, long, const int*, float*, long, hipStream_t);
 template void perm_from_tracked<double>(const double*, long, long, long, int*, hipStream_t);
 template void perm_from_tracked<float>(const float*, long, long, long, int*, hipStream_t);

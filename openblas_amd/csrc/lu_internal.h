@@ -25,6 +25,9 @@ void perm_from_tracked(const T* At, long ldt, long m, long n, int* perm_dev, hip
 
 template <typename T>
 void permute_rows(bool inverse, long n, long nrhs, const int* perm_dev, T* b, long ldb, hipStream_t s);
+// the same with a caller-provided n x nrhs staging buffer (no allocation)
+template <typename T>
+void permute_rows_ws(bool inverse, long n, long nrhs, const int* perm_dev, T* b, long ldb, T* tmp, hipStream_t s);
 
 void ipiv_to_perm(long n, const int* ipiv_host, std::vector<int>& perm);
 

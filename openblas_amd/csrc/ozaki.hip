@@ -414,6 +414,10 @@ int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const do
     oz_gemm_kernel<NSV><<<(unsigned)tiles, 256, 0, s>>>(M, N, K, Kp, sa.as<signed char>(), Mp, sb.as<signed char>(), \
                                                         Np, amax, bmax, alpha, beta, C, ldc, kchunk);             \
     break;
+  // algorithmic work: s(s+1)/2 slice products of 2MNK int8 ops each; bytes:
+  // the slices of both operands (s bytes per element) and C read + written
+  KTimer kt("ozaki_gemm", s, (double)ns * (ns + 1) / 2 * 2.0 * M * N * K,
+            (double)ns * (M * K + K * N) + 8.0 * M * N * (beta == 0.0 ? 1 : 2));
   switch (ns) {
     OZ_L(1) OZ_L(2) OZ_L(3) OZ_L(4) OZ_L(5) OZ_L(6) OZ_L(7) OZ_L(8) OZ_L(9) OZ_L(10) OZ_L(11) OZ_L(12)
   }

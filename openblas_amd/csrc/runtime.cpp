@@ -143,6 +143,72 @@ void end_call() {
   }
 }
 
+// ---- kernel timer ------------------------------------------------------------
+namespace {
+struct KRec {
+  hipEvent_t e0, e1;
+  double flops, bytes;
+};
+struct KAgg {
+  long count = 0;
+  double ms = 0, flops = 0, bytes = 0;
+  std::vector<KRec> pending;
+};
+std::mutex g_kt_mu;
+bool g_kt_on = false;
+std::map<std::string, KAgg> g_kt;
+std::vector<hipEvent_t> g_kt_pool;
+
+hipEvent_t kt_event() {
+  if (!g_kt_pool.empty()) {
+    hipEvent_t e = g_kt_pool.back();
+    g_kt_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  OBA_HIP(hipEventCreate(&e));
+  return e;
+}
+// resolve recorded pairs into the aggregates (events complete)
+void kt_resolve(KAgg& a) {
+  for (auto& r : a.pending) {
+    OBA_HIP(hipEventSynchronize(r.e1));
+    float ms = 0.f;
+    OBA_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    a.count++;
+    a.ms += ms;
+    a.flops += r.flops;
+    a.bytes += r.bytes;
+    g_kt_pool.push_back(r.e0);
+    g_kt_pool.push_back(r.e1);
+  }
+  a.pending.clear();
+}
+}  // namespace
+
+bool ktimer_on() { return g_kt_on; }
+
+KTimer::KTimer(const char* n, hipStream_t st, double fl, double by) {
+  if (!g_kt_on) return;
+  std::lock_guard<std::mutex> g(g_kt_mu);
+  name = n; s = st; flops = fl; bytes = by;
+  e0 = kt_event();
+  OBA_HIP(hipEventRecord(e0, s));
+}
+
+KTimer::~KTimer() {
+  if (!e0) return;
+  std::lock_guard<std::mutex> g(g_kt_mu);
+  hipEvent_t e1 = nullptr;
+  try {
+    e1 = kt_event();
+  } catch (const DeviceError&) {
+    return;   // never throw from a destructor; the pair is dropped
+  }
+  if (hipEventRecord(e1, s) != hipSuccess) { (void)hipGetLastError(); return; }
+  g_kt[name].pending.push_back(KRec{e0, e1, flops, bytes});
+}
+
 // ---- device pool -----------------------------------------------------------
 namespace {
 std::mutex g_pool_mu;
@@ -355,6 +421,39 @@ OBA_API float openblas_amd_timer_stop(void) {
     OBA_HIP(hipEventElapsedTime(&ms, g_t0, g_t1));
   });
   return ms;
+}
+// Kernel timer (no reference counterpart; used by the benchmark): enable or
+// disable (disabling keeps what was recorded), read one named aggregate
+// (launch count, total ms, algorithmic flops and bytes summed over the
+// launches; returns 0, or -1 if the name has no record), reset all.
+OBA_API void openblas_amd_ktimer_enable(int on) {
+  guarded([on] {
+    rt();
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    g_kt_on = on != 0;
+  });
+}
+OBA_API int openblas_amd_ktimer_read(const char* name, long* count, double* ms, double* flops, double* bytes) {
+  int r = -1;
+  guarded([&] {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    auto it = g_kt.find(name ? name : "");
+    if (it == g_kt.end()) return;
+    kt_resolve(it->second);
+    if (count) *count = it->second.count;
+    if (ms) *ms = it->second.ms;
+    if (flops) *flops = it->second.flops;
+    if (bytes) *bytes = it->second.bytes;
+    r = 0;
+  });
+  return r;
+}
+OBA_API void openblas_amd_ktimer_reset(void) {
+  guarded([] {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    for (auto& kv : g_kt) kt_resolve(kv.second);
+    g_kt.clear();
+  });
 }
 OBA_API int openblas_amd_last_xerbla(char* name, int len) {
   if (name && len > 0) { strncpy(name, g_last_name, len - 1); name[len - 1] = 0; }

@@ -218,6 +218,7 @@ void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long c
   if (rs != 1 && cs != 1) fatal("trsv_lookback: one index of A must be contiguous", nullptr);
   const long nblk = (n + TV_B - 1) / TV_B;
   unsigned* err = rt().d_err;
+  KTimer kt("trsv_lookback", s, (double)n * n, (double)sizeof(T) * (0.5 * n * n + 2.0 * n));
   if (rs == 1)
     trsv_lb_kernel<T, true><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
   else
