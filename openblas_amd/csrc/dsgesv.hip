@@ -290,16 +290,13 @@ static void sym_residual(long n, long nrhs, const double* a, long lda, int tri, 
 template <typename T>
 void potrs_device(bool lower, long n, long nrhs, const T* f, long ldf, T* b, long ldb, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return;
-  if (nrhs <= 4) {
-    for (long r = 0; r < nrhs; r++) {
-      T* x = b + r * ldb;
-      if (lower) {   // L y = b: L(i,k) = f[i + k*ldf]; L^T x = y: L^T(i,k) = f[k + i*ldf]
-        trsv_lookback<T>(true, false, n, f, 1, ldf, x, s);
-        trsv_lookback<T>(false, false, n, f, ldf, 1, x, s);
-      } else {       // U^T y = b: U^T(i,k) = f[k + i*ldf]; U x = y: U(i,k) = f[i + k*ldf]
-        trsv_lookback<T>(true, false, n, f, ldf, 1, x, s);
-        trsv_lookback<T>(false, false, n, f, 1, ldf, x, s);
-      }
+  if (nrhs <= 16) {   // one trsv_lookback launch per triangle (a chain per column)
+    if (lower) {   // L y = b: L(i,k) = f[i + k*ldf]; L^T x = y: L^T(i,k) = f[k + i*ldf]
+      trsv_lookback<T>(true, false, n, f, 1, ldf, b, ldb, nrhs, s);
+      trsv_lookback<T>(false, false, n, f, ldf, 1, b, ldb, nrhs, s);
+    } else {       // U^T y = b: U^T(i,k) = f[k + i*ldf]; U x = y: U(i,k) = f[i + k*ldf]
+      trsv_lookback<T>(true, false, n, f, ldf, 1, b, ldb, nrhs, s);
+      trsv_lookback<T>(false, false, n, f, 1, ldf, b, ldb, nrhs, s);
     }
     return;
   }

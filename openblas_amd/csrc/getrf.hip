@@ -41,7 +41,9 @@ namespace oba {
 
 constexpr int LU_W = 32;        // leaf panel width
 constexpr int LU_PMAX = 128;    // max workgroups of the panel kernel
-constexpr int LU_TRSV_MAX_RHS = 4;   // solves with this many columns or fewer use trsv_lookback
+// solves with this many columns or fewer use trsv_lookback (one chain per
+// column, all in one launch per triangle); more go to the TRSM recursion
+constexpr int LU_TRSV_MAX_RHS = 16;
 constexpr int LU_EB = 4;        // candidate record groups a wavefront polls at once
 
 
@@ -1057,27 +1059,46 @@ struct LuCtx {
 // rows per thread of lu_panel2_kernel
 constexpr int L2_RPT = 2;
 
-// which panel kernel factors the leaves: the second form unless
-// OPENBLAS_AMD_LU_PANEL=1 asks for the first (kept for comparison)
-static bool lu_use_panel2() {
+// Which panel kernel factors a leaf.  The first form (512-thread workgroups,
+// one row per thread) is the faster one up to its LU_PMAX workgroups (65536
+// rows): measured at N=16384 the FP32 factorization takes ~4.5 us per
+// column with it against ~6.2 us with the second form.  Taller panels take
+// the second form (256-thread workgroups, two rows per thread, up to
+// L2_PMAX workgroups).  OPENBLAS_AMD_LU_PANEL=2 forces the second form.
+static bool lu_force_panel2() {
   static const bool v = [] {
     const char* e = getenv("OPENBLAS_AMD_LU_PANEL");
-    return !(e && atoi(e) == 1);
+    return e && atoi(e) == 2;
   }();
   return v;
 }
 
-// Workgroups of lu_panel2_kernel that the device can hold at once: all of a
-// launch must be resident together (they exchange candidates every column).
+// Workgroups of a panel kernel that the device can hold at once.  All of a
+// launch must become resident together (they exchange candidates every
+// column); workgroups that find no free CU start as soon as the trailing
+// update's workgroups drain (that GEMM never waits for the panel), so the
+// bound is the whole device's capacity.
+template <typename K>
+static long lu_capacity(K kernel, int threads) {
+  int per_cu = 0;
+  OBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+  return (long)std::max(per_cu, 0) * rt().num_cu;
+}
+
+// The panel kernel for a leaf of mp rows (1: first form, 2: second form, 0:
+// none can hold it resident) and its workgroup count *P.
 template <typename T>
-static long lu_panel2_capacity() {
-  static long cap = -1;
-  if (cap < 0) {
-    int per_cu = 0;
-    OBA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lu_panel2_kernel<T, LU_W, L2_RPT>, L2_NT, 0));
-    cap = (long)std::max(per_cu, 0) * rt().num_cu;
-  }
-  return cap;
+static int lu_panel_form(long mp, long* P) {
+  constexpr int RPT = LuRpt<T>::v;
+  constexpr int NT = LuNt<T>::v;
+  constexpr long ROWS1 = (long)NT * RPT, ROWS2 = (long)L2_NT * L2_RPT;
+  static const long cap1 = std::min<long>(LU_PMAX, lu_capacity(lu_panel_kernel<T, LU_W, RPT, NT>, NT));
+  static const long cap2 = std::min<long>(L2_PMAX, lu_capacity(lu_panel2_kernel<T, LU_W, L2_RPT>, L2_NT));
+  const long P1 = (mp + ROWS1 - 1) / ROWS1, P2 = (mp + ROWS2 - 1) / ROWS2;
+  if (!lu_force_panel2() && P1 <= cap1) { *P = P1; return 1; }
+  if (P2 <= cap2) { *P = P2; return 2; }
+  *P = 0;
+  return 0;
 }
 
 template <typename T>
@@ -1094,21 +1115,15 @@ static void lu_panel(LuCtx<T>& c, long j0, int w, bool pre = false) {
   if (pre) lf += (double)LU_W * LU_W * w + 2.0 * mp * LU_W * w;
   {
   KTimer kt("lu_panel", c.s, lf, (double)sizeof(T) * mp * (2.0 * w + (pre ? LU_W : 0)));
-  if (lu_use_panel2()) {
-    constexpr long ROWS = (long)L2_NT * L2_RPT;
-    P = (mp + ROWS - 1) / ROWS;
-    if (P > L2_PMAX || P > lu_panel2_capacity<T>())
-      fatal("getrf: more rows than the panel kernel can hold resident at once", nullptr);
+  const int form = lu_panel_form<T>(mp, &P);
+  if (form == 1) {
+    lu_panel_kernel<T, LU_W, LuRpt<T>::v, LuNt<T>::v><<<(unsigned)P, LuNt<T>::v, 0, c.s>>>(
+        c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv, c.rec, c.base, c.err, pre ? 1 : 0);
+  } else if (form == 2) {
     lu_panel2_kernel<T, LU_W, L2_RPT><<<(unsigned)P, L2_NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
                                                                       c.rec, c.base, c.err, pre ? 1 : 0);
   } else {
-    constexpr int RPT = LuRpt<T>::v;
-    constexpr int NT = LuNt<T>::v;
-    constexpr long ROWS = (long)NT * RPT;
-    P = (mp + ROWS - 1) / ROWS;
-    if (P > LU_PMAX) fatal("getrf: more rows than the panel kernel supports", nullptr);
-    lu_panel_kernel<T, LU_W, RPT, NT><<<(unsigned)P, NT, 0, c.s>>>(c.At, c.ldt, c.m, j0, w, c.ipiv, c.info, mv,
-                                                                          c.rec, c.base, c.err, pre ? 1 : 0);
+    fatal("getrf: more rows than the panel kernels can hold resident at once", nullptr);
   }
   OBA_HIP(hipGetLastError());
   }
@@ -1235,6 +1250,9 @@ template <typename T>
 int getrf_t(long m, long n, T* At, long ldt, int* ipiv_dev, bool track_perm, hipStream_t s) {
   const long mn = std::min(m, n);
   if (mn <= 0) return 0;
+  long p_first = 0;   // the first leaf is the tallest: reject before any work is queued
+  if (lu_panel_form<T>(m, &p_first) == 0)
+    fatal("getrf: more rows than the panel kernels can hold resident at once", nullptr);
   const long npanels = (mn + LU_W - 1) / LU_W + 1;
   const size_t rec_bytes = sizeof(unsigned long long) * LuRec<T, LU_W>::RW * 2 * std::max(LU_PMAX, L2_PMAX);
   const size_t mv_bytes = sizeof(int) * npanels * LU_MV;
@@ -1345,24 +1363,20 @@ template <typename T>
 void getrs_perm(bool trans, long n, long nrhs, const T* a, long lda, const int* perm_dev, T* b, long ldb,
                 hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return;
-  const bool few = nrhs <= LU_TRSV_MAX_RHS;   // one launch per triangle and column
+  const bool few = nrhs <= LU_TRSV_MAX_RHS;   // one launch per triangle
   if (!trans) {
     permute_rows<T>(false, n, nrhs, perm_dev, b, ldb, s);
     if (few) {
-      for (long r = 0; r < nrhs; r++) {
-        trsv_lookback<T>(true, true, n, a, 1, lda, b + r * ldb, s);
-        trsv_lookback<T>(false, false, n, a, 1, lda, b + r * ldb, s);
-      }
+      trsv_lookback<T>(true, true, n, a, 1, lda, b, ldb, nrhs, s);
+      trsv_lookback<T>(false, false, n, a, 1, lda, b, ldb, nrhs, s);
     } else {
       trsm<T>(false, true, false, true, n, nrhs, T(1), a, lda, b, ldb, s);
       trsm<T>(false, false, false, false, n, nrhs, T(1), a, lda, b, ldb, s);
     }
   } else {
-    if (few) {
-      for (long r = 0; r < nrhs; r++) {   // U^T y = b (lower), then L^T x = y (upper, unit)
-        trsv_lookback<T>(true, false, n, a, lda, 1, b + r * ldb, s);
-        trsv_lookback<T>(false, true, n, a, lda, 1, b + r * ldb, s);
-      }
+    if (few) {   // U^T y = b (lower), then L^T x = y (upper, unit)
+      trsv_lookback<T>(true, false, n, a, lda, 1, b, ldb, nrhs, s);
+      trsv_lookback<T>(false, true, n, a, lda, 1, b, ldb, nrhs, s);
     } else {
       trsm<T>(false, false, true, false, n, nrhs, T(1), a, lda, b, ldb, s);
       trsm<T>(false, true, true, true, n, nrhs, T(1), a, lda, b, ldb, s);
@@ -1377,11 +1391,9 @@ void getrs_t(long n, long nrhs, const T* At, long ldt, const int* perm_dev, T* b
   if (n <= 0 || nrhs <= 0) return;
   permute_rows<T>(false, n, nrhs, perm_dev, b, ldb, s);
   // L y = b: L = (unit upper part of At)^T ; U x = y: U = (lower part of At)^T
-  if (nrhs <= LU_TRSV_MAX_RHS) {
-    for (long r = 0; r < nrhs; r++) {   // L(i,k) = At[k + i*ldt], U(i,k) = At[k + i*ldt]
-      trsv_lookback<T>(true, true, n, At, ldt, 1, b + r * ldb, s);
-      trsv_lookback<T>(false, false, n, At, ldt, 1, b + r * ldb, s);
-    }
+  if (nrhs <= LU_TRSV_MAX_RHS) {   // L(i,k) = At[k + i*ldt], U(i,k) = At[k + i*ldt]
+    trsv_lookback<T>(true, true, n, At, ldt, 1, b, ldb, nrhs, s);
+    trsv_lookback<T>(false, false, n, At, ldt, 1, b, ldb, nrhs, s);
     return;
   }
   trsm<T>(false, false, true, true, n, nrhs, T(1), At, ldt, b, ldb, s);

@@ -35,18 +35,20 @@ template <typename T>
 void getrs_perm(bool trans, long n, long nrhs, const T* a, long lda, const int* perm_dev, T* b, long ldb,
                 hipStream_t s);
 
-// One right-hand side: M x = b in a single launch, M(i,k) = a[i*rs + k*cs]
-// (rs == 1 or cs == 1), lower -> forward, upper -> backward substitution.
+// M X = B in a single launch for the ncols columns X = x + c*ldx, M(i,k) =
+// a[i*rs + k*cs] (rs == 1 or cs == 1), lower -> forward, upper -> backward
+// substitution; one dependency chain per column, all chains side by side.
 template <typename T>
-void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, hipStream_t s);
+void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, long ldx, long ncols,
+                   hipStream_t s);
 
-// The same on caller-provided workspace: `ticket` (one zeroed word) and `pub`
-// (trsv_lookback_ws_bytes(n) - 256 zeroed bytes); no allocation.
+// The same on caller-provided workspace of trsv_lookback_ws_bytes(n, ncols)
+// zeroed bytes; no allocation.
 template <typename T>
-size_t trsv_lookback_ws_bytes(long n);
+size_t trsv_lookback_ws_bytes(long n, long ncols);
 template <typename T>
-void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, unsigned long long* pub,
-                      unsigned* ticket, hipStream_t s);
+void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, long ldx, long ncols,
+                      void* ws, hipStream_t s);
 
 template <typename T>
 void getrs_t(long n, long nrhs, const T* At, long ldt, const int* perm_dev, T* b, long ldb, hipStream_t s);

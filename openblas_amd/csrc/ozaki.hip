@@ -34,63 +34,80 @@ constexpr int OZ_NONFINITE = -2;   // return code: NaN/Inf operand, native GEMM 
 constexpr int OZ_BK = 64;       // k per slab (one i8 MFMA k-depth)
 constexpr int OZ_T = 64;        // C tile edge
 
+// Per-vector statistics: the largest |x| (it sets the vector's scale) and,
+// over the nonzero entries, the sum of log2|x| and their count (their
+// geometric mean says how far below the scale the typical entry sits, which
+// is what the slice count has to cover).
+// log2|x| to about 0.09 (exponent plus the linear mantissa term): enough for
+// a slice count, and no transcendental call per element
+__device__ inline double oz_log2_approx(double a) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(a);
+  const int ex = (int)((b >> 52) & 0x7ff) - 1023;
+  const double m = (double)(b & 0xFFFFFFFFFFFFFull) * 0x1p-52;
+  return (double)ex + m;
+}
+
 // element (v, k) of an operand "vector" v at X[v*sv + k*sk]
 // sv == 1: the k range is cut into gridDim.y chunks (one vector per thread
-// and chunk, loads coalesced across the vectors); partial max/sum per chunk,
-// combined in chunk order by oz_stats_combine (deterministic)
+// and chunk, loads coalesced across the vectors); partial statistics per
+// chunk, combined in chunk order by oz_stats_combine (deterministic)
 // A NaN or infinity anywhere raises *nonfinite: the slices cannot carry it
 // (the host then runs the native FP64 GEMM, which propagates it).
-__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* pmax, double* psum,
-                                 int* nonfinite) {
+__global__ void oz_stats_vcontig(long V, long K, const double* __restrict__ X, long sk, double* pmax, double* plog,
+                                 double* pcnt, int* nonfinite) {
   const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (v >= V) return;
   const long kc = (K + gridDim.y - 1) / gridDim.y, k0 = blockIdx.y * kc, k1 = min(K, k0 + kc);
-  double mx = 0.0, sm = 0.0;
+  double mx = 0.0, lg = 0.0, cn = 0.0;
   bool bad = false;
   for (long k = k0; k < k1; k++) {
     double a = fabs(X[v + k * sk]);
     bad |= !(a <= DBL_MAX);
     mx = a > mx ? a : mx;
-    sm += a;
+    if (a > 0.0) { lg += oz_log2_approx(a); cn += 1.0; }
   }
   if (bad) atomicOr(nonfinite, 1);
   pmax[blockIdx.y * V + v] = mx;
-  psum[blockIdx.y * V + v] = sm;
+  plog[blockIdx.y * V + v] = lg;
+  pcnt[blockIdx.y * V + v] = cn;
 }
 
-__global__ void oz_stats_combine(long V, int parts, const double* __restrict__ pmax, const double* __restrict__ psum,
-                                 double* vmax, double* vsum) {
+__global__ void oz_stats_combine(long V, int parts, const double* __restrict__ pmax, const double* __restrict__ plog,
+                                 const double* __restrict__ pcnt, double* vmax, double* vlog, double* vcnt) {
   const long v = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (v >= V) return;
-  double mx = 0.0, sm = 0.0;
+  double mx = 0.0, lg = 0.0, cn = 0.0;
   for (int p = 0; p < parts; p++) {
     const double a = pmax[p * V + v];
     mx = a > mx ? a : mx;
-    sm += psum[p * V + v];
+    lg += plog[p * V + v];
+    cn += pcnt[p * V + v];
   }
   vmax[v] = mx;
-  vsum[v] = sm;
+  vlog[v] = lg;
+  vcnt[v] = cn;
 }
 
-__global__ void oz_stats_kcontig(long V, long K, const double* __restrict__ X, long sv, double* vmax, double* vsum,
-                                 int* nonfinite) {
+__global__ void oz_stats_kcontig(long V, long K, const double* __restrict__ X, long sv, double* vmax, double* vlog,
+                                 double* vcnt, int* nonfinite) {
   const long v = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6);   // sk == 1, one wave per vector
   const int lane = threadIdx.x & 63;
   if (v >= V) return;
-  double mx = 0.0, sm = 0.0;
+  double mx = 0.0, lg = 0.0, cn = 0.0;
   bool bad = false;
   for (long k = lane; k < K; k += 64) {
     double a = fabs(X[v * sv + k]);
     bad |= !(a <= DBL_MAX);
     mx = a > mx ? a : mx;
-    sm += a;
+    if (a > 0.0) { lg += oz_log2_approx(a); cn += 1.0; }
   }
   if (bad) atomicOr(nonfinite, 1);
   for (int off = 32; off > 0; off >>= 1) {
     mx = fmax(mx, __shfl_xor(mx, off));
-    sm += __shfl_xor(sm, off);
+    lg += __shfl_xor(lg, off);
+    cn += __shfl_xor(cn, off);
   }
-  if (lane == 0) { vmax[v] = mx; vsum[v] = sm; }
+  if (lane == 0) { vmax[v] = mx; vlog[v] = lg; vcnt[v] = cn; }
 }
 
 // exponent e with max*2^-e < 1 (max = f*2^e, f in [0.5,1)); 0 for max == 0
@@ -320,39 +337,47 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(long M, long N, long K, lo
   }
 }
 
-static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vsum, int* nonfinite,
-                     hipStream_t s) {
+static void oz_stats(long V, long K, const double* X, long sv, long sk, double* vmax, double* vlog, double* vcnt,
+                     int* nonfinite, hipStream_t s) {
   if (V <= 0) return;
   if (sv == 1) {
     // enough workgroups to fill the device: ~1024 (vector blocks x k chunks)
     const long vb = (V + 255) / 256;
     const int parts = (int)std::max(1L, std::min((K + 63) / 64, 1024 / vb));
-    DevBuf part(sizeof(double) * 2 * (size_t)parts * V);
+    DevBuf part(sizeof(double) * 3 * (size_t)parts * V);
     double* pmax = part.as<double>();
-    double* psum = pmax + (size_t)parts * V;
-    oz_stats_vcontig<<<dim3((unsigned)vb, (unsigned)parts), 256, 0, s>>>(V, K, X, sk, pmax, psum, nonfinite);
+    double* plog = pmax + (size_t)parts * V;
+    double* pcnt = plog + (size_t)parts * V;
+    oz_stats_vcontig<<<dim3((unsigned)vb, (unsigned)parts), 256, 0, s>>>(V, K, X, sk, pmax, plog, pcnt, nonfinite);
     OBA_HIP(hipGetLastError());
-    oz_stats_combine<<<(unsigned)vb, 256, 0, s>>>(V, parts, pmax, psum, vmax, vsum);
+    oz_stats_combine<<<(unsigned)vb, 256, 0, s>>>(V, parts, pmax, plog, pcnt, vmax, vlog, vcnt);
   } else {
-    oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vsum, nonfinite);
+    oz_stats_kcontig<<<(unsigned)((V + 3) / 4), 256, 0, s>>>(V, K, X, sv, vmax, vlog, vcnt, nonfinite);
   }
   OBA_HIP(hipGetLastError());
 }
 
-int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* asum, const double* bmax,
-                      const double* bsum) {
-  // peakedness of rows/columns: log2(max / mean |x|); each bit of it costs
-  // accuracy of the per-vector scaled digits of the small entries
-  auto peak = [K](long V, const double* mx, const double* sm) {
+int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* alog, const double* acnt,
+                      const double* bmax, const double* blog, const double* bcnt) {
+  // Spread of a row (column): log2 of its largest |x| over the geometric
+  // mean of its nonzero |x|.  The digits are cut at a fixed distance below
+  // the vector's largest entry, so an entry that sits d bits below it keeps
+  // d bits fewer; the typical entry of a row times the typical entry of a
+  // column is what C's components are made of, and 53 bits of it must
+  // survive.  (The arithmetic mean is dominated by the largest entries of a
+  // heavy-tailed vector and underestimates the spread: exp(8 N(0,1))-scaled
+  // data then got 12 slices and a 2e-9 result.)
+  (void)K;
+  auto spread = [](long V, const double* mx, const double* lg, const double* cn) {
     double worst = 0.0;
     for (long v = 0; v < V; v++) {
-      if (mx[v] <= 0.0 || sm[v] <= 0.0) continue;
-      double r = std::log2(mx[v] * (double)K / sm[v]);
+      if (mx[v] <= 0.0 || cn[v] <= 0.0) continue;
+      const double r = std::log2(mx[v]) - lg[v] / cn[v];
       if (r > worst) worst = r;
     }
     return worst;
   };
-  const double bits = 53.0 + peak(M, amax, asum) + peak(N, bmax, bsum);
+  const double bits = 53.0 + spread(M, amax, alog, acnt) + spread(N, bmax, blog, bcnt);
   const double s = std::ceil(bits / 7.0);
   return (int)std::max(2.0, std::min(s, 1000.0));   // the required count, not clamped to OZ_SMAX
 }
@@ -365,20 +390,23 @@ int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const do
   const long a_sv = ta ? lda : 1, a_sk = ta ? 1 : lda;
   // op(B) column j, element k: B[k + j*ldb] (N) or B[j + k*ldb] (T)
   const long b_sv = tb ? 1 : ldb, b_sk = tb ? ldb : 1;
-  DevBuf st(sizeof(double) * 2 * (M + N) + 256);
+  // statistics: amax, alog, acnt (M each), then bmax, blog, bcnt (N each)
+  DevBuf st(sizeof(double) * 3 * (M + N) + 256);
   double* amax = st.as<double>();
-  double* asum = amax + M;
-  double* bmax = asum + M;
-  double* bsum = bmax + N;
-  int* nonfinite = (int*)(bsum + N);
+  double* alog = amax + M;
+  double* acnt = alog + M;
+  double* bmax = acnt + M;
+  double* blog = bmax + N;
+  double* bcnt = blog + N;
+  int* nonfinite = (int*)(bcnt + N);
   OBA_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
-  oz_stats(M, K, A, a_sv, a_sk, amax, asum, nonfinite, s);
-  oz_stats(N, K, B, b_sv, b_sk, bmax, bsum, nonfinite, s);
+  oz_stats(M, K, A, a_sv, a_sk, amax, alog, acnt, nonfinite, s);
+  oz_stats(N, K, B, b_sv, b_sk, bmax, blog, bcnt, nonfinite, s);
   int ns = slices;
   int h_nonfinite = 0;
-  std::vector<double> h(ns <= 0 ? 2 * (M + N) : 0);
+  std::vector<double> h(ns <= 0 ? 3 * (M + N) : 0);
   OBA_HIP(hipMemcpyAsync(&h_nonfinite, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
-  if (ns <= 0) OBA_HIP(hipMemcpyAsync(h.data(), amax, sizeof(double) * 2 * (M + N), hipMemcpyDeviceToHost, s));
+  if (ns <= 0) OBA_HIP(hipMemcpyAsync(h.data(), amax, sizeof(double) * 3 * (M + N), hipMemcpyDeviceToHost, s));
   OBA_HIP(hipStreamSynchronize(s));
   if (h_nonfinite) {
     // NaN / Inf: the native FP64 GEMM gives cblas_dgemm's propagation
@@ -386,7 +414,8 @@ int ozaki_dgemm(bool ta, bool tb, long M, long N, long K, double alpha, const do
     return OZ_NONFINITE;
   }
   if (ns <= 0) {
-    ns = ozaki_slices_auto(M, N, K, h.data(), h.data() + M, h.data() + 2 * M, h.data() + 2 * M + N);
+    const double* hb = h.data() + 3 * M;
+    ns = ozaki_slices_auto(M, N, K, h.data(), h.data() + M, h.data() + 2 * M, hb, hb + N, hb + 2 * N);
     if (ns > OZ_SMAX) {
       // the exponent spread needs more digits than the fused kernel holds: a
       // 12-slice result would miss FP64 accuracy, so the native FP64 GEMM runs

@@ -1,6 +1,6 @@
-// Triangular solve with one right-hand side in one launch (the few-RHS case
-// of the LU solves, lapack/getrs/getrs_single.c, which the reference runs as
-// TRSV for NRHS = 1).
+// Triangular solves with few right-hand sides in one launch (the few-RHS
+// case of the LU solves, lapack/getrs/getrs_single.c, which the reference
+// runs as TRSV for NRHS = 1).
 //
 // M x = b with M(i,k) = A[i*rs + k*cs] (lower: forward substitution, upper:
 // backward), unit or non-unit diagonal.  The rows are cut into 64-row
@@ -37,11 +37,18 @@ __device__ inline float tv_bcast(float v, int l) { return __int_as_float(__built
 // published x of a block: TV_B values as 32-bit words, each {word, tag 1}
 template <typename T> struct TvPub { static constexpr int WPV = sizeof(T) / 4; };
 
+// Several right-hand sides: column c = blockIdx.y is x + c*ldx with its own
+// ticket counter ticket[c] and published words pub + c*pstride, i.e. an
+// independent chain per column; the chains run side by side in one launch.
 template <typename T, bool ROWLANES>
 __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restrict__ A, long rs, long cs, bool lower,
-                                                      bool unit, T* __restrict__ x, unsigned long long* __restrict__ pub,
+                                                      bool unit, T* __restrict__ x, long ldx,
+                                                      unsigned long long* __restrict__ pub, long pstride,
                                                       unsigned* __restrict__ ticket, unsigned* __restrict__ err) {
   constexpr int WPV = TvPub<T>::WPV;
+  x += (long)blockIdx.y * ldx;
+  pub += (long)blockIdx.y * pstride;
+  ticket += blockIdx.y;
   constexpr int LD = TV_B + 1;
   __shared__ T Md[TV_B * LD];      // M(r0+i, r0+k) at Md[i*LD + k]
   __shared__ T part[4][TV_B];
@@ -189,50 +196,60 @@ __global__ __launch_bounds__(256) void trsv_lb_kernel(long n, const T* __restric
   }
 }
 
+// published words of one column's chain (bytes)
 template <typename T>
-void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, hipStream_t s) {
-  if (n <= 0) return;
-  if (rs != 1 && cs != 1) fatal("trsv_lookback: one index of A must be contiguous", nullptr);
-  const long nblk = (n + TV_B - 1) / TV_B;
-  const size_t pub_bytes = sizeof(unsigned long long) * (size_t)nblk * TV_B * TvPub<T>::WPV;
-  DevBuf ws(256 + pub_bytes);
-  unsigned* ticket = ws.as<unsigned>();
-  unsigned long long* pub = (unsigned long long*)(ws.as<char>() + 256);
-  OBA_HIP(hipMemsetAsync(ws.p, 0, 256 + pub_bytes, s));
-  trsv_lookback_ws<T>(lower, unit, n, a, rs, cs, x, pub, ticket, s);
+static size_t tv_pub_bytes(long n) {
+  return sizeof(unsigned long long) * (size_t)((n + TV_B - 1) / TV_B) * TV_B * TvPub<T>::WPV;
+}
+static size_t tv_ticket_bytes(long ncols) { return ((size_t)ncols * sizeof(unsigned) + 255) / 256 * 256; }
+
+template <typename T>
+size_t trsv_lookback_ws_bytes(long n, long ncols) {
+  return tv_ticket_bytes(ncols) + (size_t)ncols * tv_pub_bytes<T>(n);
 }
 
 template <typename T>
-size_t trsv_lookback_ws_bytes(long n) {
-  return 256 + sizeof(unsigned long long) * (size_t)((n + TV_B - 1) / TV_B) * TV_B * TvPub<T>::WPV;
+void trsv_lookback(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, long ldx, long ncols,
+                   hipStream_t s) {
+  if (n <= 0 || ncols <= 0) return;
+  const size_t bytes = trsv_lookback_ws_bytes<T>(n, ncols);
+  DevBuf ws(bytes);
+  OBA_HIP(hipMemsetAsync(ws.p, 0, bytes, s));
+  trsv_lookback_ws<T>(lower, unit, n, a, rs, cs, x, ldx, ncols, ws.p, s);
 }
 
-// The launch on caller-provided, zeroed workspace (ticket: 1 word, pub:
-// trsv_lookback_ws_bytes(n) - 256 bytes).  A wait that runs out sets the
-// runtime's device error word, which the API layer checks after its next
-// synchronization (the kernel itself never blocks the host).
+// The launch on caller-provided, zeroed workspace of trsv_lookback_ws_bytes(n,
+// ncols) bytes (the ticket words, then each column's published words).  A
+// wait that runs out sets the runtime's device error word, which the API
+// layer checks after its next synchronization (the kernel itself never
+// blocks the host).
 template <typename T>
-void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, unsigned long long* pub,
-                      unsigned* ticket, hipStream_t s) {
-  if (n <= 0) return;
+void trsv_lookback_ws(bool lower, bool unit, long n, const T* a, long rs, long cs, T* x, long ldx, long ncols,
+                      void* ws, hipStream_t s) {
+  if (n <= 0 || ncols <= 0) return;
   if (rs != 1 && cs != 1) fatal("trsv_lookback: one index of A must be contiguous", nullptr);
+  if (ncols > 65535) fatal("trsv_lookback: too many right-hand sides for one launch", nullptr);
   const long nblk = (n + TV_B - 1) / TV_B;
+  unsigned* ticket = (unsigned*)ws;
+  unsigned long long* pub = (unsigned long long*)((char*)ws + tv_ticket_bytes(ncols));
+  const long pstride = (long)(tv_pub_bytes<T>(n) / sizeof(unsigned long long));
   unsigned* err = rt().d_err;
-  KTimer kt("trsv_lookback", s, (double)n * n, (double)sizeof(T) * (0.5 * n * n + 2.0 * n));
+  KTimer kt("trsv_lookback", s, (double)n * n * ncols, (double)sizeof(T) * (0.5 * n * n + 2.0 * n * ncols));
+  const dim3 grid((unsigned)nblk, (unsigned)ncols);
   if (rs == 1)
-    trsv_lb_kernel<T, true><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
+    trsv_lb_kernel<T, true><<<grid, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, ldx, pub, pstride, ticket, err);
   else
-    trsv_lb_kernel<T, false><<<(unsigned)nblk, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, pub, ticket, err);
+    trsv_lb_kernel<T, false><<<grid, 256, 0, s>>>(n, a, rs, cs, lower, unit, x, ldx, pub, pstride, ticket, err);
   OBA_HIP(hipGetLastError());
 }
 
-template size_t trsv_lookback_ws_bytes<double>(long);
-template size_t trsv_lookback_ws_bytes<float>(long);
-template void trsv_lookback_ws<double>(bool, bool, long, const double*, long, long, double*, unsigned long long*,
-                                       unsigned*, hipStream_t);
-template void trsv_lookback_ws<float>(bool, bool, long, const float*, long, long, float*, unsigned long long*,
-                                      unsigned*, hipStream_t);
-template void trsv_lookback<double>(bool, bool, long, const double*, long, long, double*, hipStream_t);
-template void trsv_lookback<float>(bool, bool, long, const float*, long, long, float*, hipStream_t);
+template size_t trsv_lookback_ws_bytes<double>(long, long);
+template size_t trsv_lookback_ws_bytes<float>(long, long);
+template void trsv_lookback_ws<double>(bool, bool, long, const double*, long, long, double*, long, long, void*,
+                                       hipStream_t);
+template void trsv_lookback_ws<float>(bool, bool, long, const float*, long, long, float*, long, long, void*,
+                                      hipStream_t);
+template void trsv_lookback<double>(bool, bool, long, const double*, long, long, double*, long, long, hipStream_t);
+template void trsv_lookback<float>(bool, bool, long, const float*, long, long, float*, long, long, hipStream_t);
 
 }  // namespace oba

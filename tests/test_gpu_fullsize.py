@@ -9,8 +9,9 @@ ABI, with size-independent properties.
   (LAPACK dget04's ratio, threshold 30); ITER follows the ddrvab.f:354-364
   rule: ITER >= 0 requires ||b - A x|| / (||A|| ||x|| eps) < sqrt(N), ITER < 0
   (the FP64 fallback of dsgesv.f:405-423) requires it < 30.  For kappa <= 1e6
-  the refinement must converge (ITER >= 0); for kappa = 1e10 (FP32 LU far
-  beyond its accuracy) it must take the fallback.
+  the refinement must converge (ITER >= 0); for kappa = 1e10 (kappa * eps32
+  ~ 600) it may converge slowly or give up (ITER = -31), depending on the
+  FP32 factor's rounding.
 * Batched small problems, 10,000 problems with n ~ U{8..256}, FP64 and FP32:
   every GEMM passes the ctest dmmch ratio, every getrf the dget01
   reconstruction ratio, every getrs the dget02 residual ratio, every potrf
@@ -96,10 +97,15 @@ def test_c3_solve_16384(oba, system16k, nrhs, mixed):
     assert np.all(fwd <= 30 * kappa * EPS64), (fwd.max(), kappa)
     if mixed:
         ratio = float((rn / (anorm * xn * EPS64)).max())   # dget08's test ratio
+        # kappa <= 1e6 is well inside what an FP32 LU can refine.  At kappa =
+        # 1e10 (kappa * eps32 ~ 600) refinement may still converge or may
+        # give up after ITERMAX sweeps (ITER = -31, FP64 fallback): which one
+        # depends on the FP32 factor's rounding, so the outcome is not fixed,
+        # and lapack-netlib/TESTING/LIN/ddrvab.f:354-364 checks the solution
+        # by the branch taken, as below.
         if kappa <= 1e6:
             assert it >= 0
-        else:
-            assert it < 0   # the FP32 LU cannot refine kappa = 1e10: FP64 fallback
+        assert it >= 0 or it == -31
         if it >= 0:
             assert ratio < np.sqrt(n)
         else:
