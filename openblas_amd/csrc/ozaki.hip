@@ -363,10 +363,14 @@ int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* 
   // mean of its nonzero |x|.  The digits are cut at a fixed distance below
   // the vector's largest entry, so an entry that sits d bits below it keeps
   // d bits fewer; the typical entry of a row times the typical entry of a
-  // column is what C's components are made of, and 53 bits of it must
-  // survive.  (The arithmetic mean is dominated by the largest entries of a
-  // heavy-tailed vector and underestimates the spread: exp(8 N(0,1))-scaled
-  // data then got 12 slices and a 2e-9 result.)
+  // column is what C's components are made of; 50 bits of it are kept,
+  // which holds the result at the accuracy of a native FP64 product's own
+  // rounding (8192^3, entries u*exp(phi*g): phi = 0.5 -> 9 slices, error
+  // 7e-17 against native 5.5e-16; phi = 2 -> 12 slices, 3e-15 against
+  // 1.2e-14; phi = 8 needs 26 and takes the native path).  The arithmetic
+  // mean is dominated by the largest entries of a heavy-tailed vector and
+  // underestimates the spread (it gave phi = 8 data 12 slices and a 2e-9
+  // result).
   (void)K;
   auto spread = [](long V, const double* mx, const double* lg, const double* cn) {
     double worst = 0.0;
@@ -377,7 +381,7 @@ int ozaki_slices_auto(long M, long N, long K, const double* amax, const double* 
     }
     return worst;
   };
-  const double bits = 53.0 + spread(M, amax, alog, acnt) + spread(N, bmax, blog, bcnt);
+  const double bits = 50.0 + spread(M, amax, alog, acnt) + spread(N, bmax, blog, bcnt);
   const double s = std::ceil(bits / 7.0);
   return (int)std::max(2.0, std::min(s, 1000.0));   // the required count, not clamped to OZ_SMAX
 }

@@ -92,7 +92,10 @@ def test_c3_solve_16384(oba, system16k, nrhs, mixed):
     xn = np.abs(x).max(axis=0)
     rn = np.abs(r).max(axis=0)
     bwd = rn / (anorm * xn + np.abs(b).max(axis=0))
-    assert np.all(bwd <= 1e-14), bwd.max()
+    # FP64-level backward error; refinement that converges slowly (kappa =
+    # 1e10) stops at dsgesv.f's own test, ||r|| <= ||x|| ||A|| eps sqrt(N)
+    slow = mixed and kappa > 1e6
+    assert np.all(bwd <= (np.sqrt(n) * EPS64 if slow else 1e-14)), bwd.max()
     fwd = np.abs(x - xt).max(axis=0) / np.abs(xt).max(axis=0)
     assert np.all(fwd <= 30 * kappa * EPS64), (fwd.max(), kappa)
     if mixed:
