@@ -232,6 +232,18 @@ float openblas_amd_timer_stop(void);
 void openblas_amd_ktimer_enable(int on);
 int openblas_amd_ktimer_read(const char *name, long *count, double *ms, double *flops, double *bytes);
 void openblas_amd_ktimer_reset(void);
+/* dgetrs_ that allocates nothing (no counterpart in the reference): for
+   callers that cannot allocate device memory -- code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+   arguments, checks and result as dgetrs_ (the same kernels), but A and B
+   must be device memory (IPIV host or device), N <= 32768, and the kernels'
+   workspace is a 6.1 MiB static device array per GPU instead of the memory
+   pool.  Concurrent calls on one GPU race on that array.  A host A or B, or
+   N > 32768, is reported through xerbla ("DGETRS_NOALLOC", parameter 4, 7
+   or 2) with INFO = -4, -7 or -2. */
+void openblas_amd_dgetrs_noalloc(const char *trans, const blasint *n, const blasint *nrhs, const double *a,
+                                 const blasint *lda, const blasint *ipiv, double *b, const blasint *ldb,
+                                 blasint *info);
 /* last argument error reported through xerbla on this thread (0 if none);
    copies the routine name; clears the record */
 int openblas_amd_last_xerbla(char *name, int len);

@@ -45,6 +45,7 @@ _SIGS = {
     "dgetf2_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pi]),
     "sgetf2_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pi]),
     "dgetrs_": (c_int, [c_char_p, _pi, _pi, _pd, _pi, c_void_p, _pd, _pi, _pi]),
+    "openblas_amd_dgetrs_noalloc": (None, [c_char_p, _pi, _pi, _pd, _pi, c_void_p, _pd, _pi, _pi]),
     "sgetrs_": (c_int, [c_char_p, _pi, _pi, _pd, _pi, c_void_p, _pd, _pi, _pi]),
     "dgesv_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pd, _pi, _pi]),
     "sgesv_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pd, _pi, _pi]),
