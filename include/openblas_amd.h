@@ -74,6 +74,31 @@ void cblas_sgemm_batch(enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE *Trans
                        const blasint *ldb_array, const float *beta_array, float **C_array,
                        const blasint *ldc_array, blasint group_count, const blasint *group_size);
 
+/* cblas.h:313 (gemmt), :322 (symm), :331 (syrk), :349 (trmm); interface/syrk.c,
+   symm.c, trsm.c (TRMM), gemmt.c built with -DCBLAS */
+void cblas_dsyrk(enum CBLAS_ORDER Order, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE Trans, blasint N, blasint K,
+                 double alpha, const double *A, blasint lda, double beta, double *C, blasint ldc);
+void cblas_ssyrk(enum CBLAS_ORDER Order, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE Trans, blasint N, blasint K,
+                 float alpha, const float *A, blasint lda, float beta, float *C, blasint ldc);
+void cblas_dgemmt(enum CBLAS_ORDER Order, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE TransA,
+                  enum CBLAS_TRANSPOSE TransB, blasint M, blasint K, double alpha, const double *A, blasint lda,
+                  const double *B, blasint ldb, double beta, double *C, blasint ldc);
+void cblas_sgemmt(enum CBLAS_ORDER Order, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE TransA,
+                  enum CBLAS_TRANSPOSE TransB, blasint M, blasint K, float alpha, const float *A, blasint lda,
+                  const float *B, blasint ldb, float beta, float *C, blasint ldc);
+void cblas_dsymm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo, blasint M, blasint N,
+                 double alpha, const double *A, blasint lda, const double *B, blasint ldb, double beta, double *C,
+                 blasint ldc);
+void cblas_ssymm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo, blasint M, blasint N,
+                 float alpha, const float *A, blasint lda, const float *B, blasint ldb, float beta, float *C,
+                 blasint ldc);
+void cblas_dtrmm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE TransA,
+                 enum CBLAS_DIAG Diag, blasint M, blasint N, double alpha, const double *A, blasint lda, double *B,
+                 blasint ldb);
+void cblas_strmm(enum CBLAS_ORDER Order, enum CBLAS_SIDE Side, enum CBLAS_UPLO Uplo, enum CBLAS_TRANSPOSE TransA,
+                 enum CBLAS_DIAG Diag, blasint M, blasint N, float alpha, const float *A, blasint lda, float *B,
+                 blasint ldb);
+
 /* ---- Fortran BLAS level 3 (BLASFUNC(x) = x##_, common.h:194) -------------- */
 /* interface/gemm.c:181 */
 void dgemm_(const char *transa, const char *transb, const blasint *m, const blasint *n, const blasint *k,
@@ -87,6 +112,30 @@ void dtrsm_(const char *side, const char *uplo, const char *transa, const char *
             const blasint *n, const double *alpha, const double *a, const blasint *lda, double *b,
             const blasint *ldb);
 void strsm_(const char *side, const char *uplo, const char *transa, const char *diag, const blasint *m,
+            const blasint *n, const float *alpha, const float *a, const blasint *lda, float *b,
+            const blasint *ldb);
+
+/* interface/syrk.c, gemmt.c, symm.c, trsm.c (TRMM) */
+void dsyrk_(const char *uplo, const char *trans, const blasint *n, const blasint *k, const double *alpha,
+            const double *a, const blasint *lda, const double *beta, double *c, const blasint *ldc);
+void ssyrk_(const char *uplo, const char *trans, const blasint *n, const blasint *k, const float *alpha,
+            const float *a, const blasint *lda, const float *beta, float *c, const blasint *ldc);
+void dgemmt_(const char *uplo, const char *transa, const char *transb, const blasint *m, const blasint *k,
+             const double *alpha, const double *a, const blasint *lda, const double *b, const blasint *ldb,
+             const double *beta, double *c, const blasint *ldc);
+void sgemmt_(const char *uplo, const char *transa, const char *transb, const blasint *m, const blasint *k,
+             const float *alpha, const float *a, const blasint *lda, const float *b, const blasint *ldb,
+             const float *beta, float *c, const blasint *ldc);
+void dsymm_(const char *side, const char *uplo, const blasint *m, const blasint *n, const double *alpha,
+            const double *a, const blasint *lda, const double *b, const blasint *ldb, const double *beta, double *c,
+            const blasint *ldc);
+void ssymm_(const char *side, const char *uplo, const blasint *m, const blasint *n, const float *alpha,
+            const float *a, const blasint *lda, const float *b, const blasint *ldb, const float *beta, float *c,
+            const blasint *ldc);
+void dtrmm_(const char *side, const char *uplo, const char *transa, const char *diag, const blasint *m,
+            const blasint *n, const double *alpha, const double *a, const blasint *lda, double *b,
+            const blasint *ldb);
+void strmm_(const char *side, const char *uplo, const char *transa, const char *diag, const blasint *m,
             const blasint *n, const float *alpha, const float *a, const blasint *lda, float *b,
             const blasint *ldb);
 

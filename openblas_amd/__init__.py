@@ -40,6 +40,22 @@ _SIGS = {
     "sgemm_": (None, [c_char_p, c_char_p, _pi, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi, POINTER(c_float), _pd, _pi]),
     "dtrsm_": (None, [c_char_p, c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi]),
     "strsm_": (None, [c_char_p, c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi]),
+    "cblas_dsyrk": (None, [c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, c_double, _pd, c_int]),
+    "cblas_ssyrk": (None, [c_int, c_int, c_int, c_int, c_int, c_float, _pd, c_int, c_float, _pd, c_int]),
+    "cblas_dgemmt": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int, c_double, _pd, c_int]),
+    "cblas_sgemmt": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, _pd, c_int, _pd, c_int, c_float, _pd, c_int]),
+    "cblas_dsymm": (None, [c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int, c_double, _pd, c_int]),
+    "cblas_ssymm": (None, [c_int, c_int, c_int, c_int, c_int, c_float, _pd, c_int, _pd, c_int, c_float, _pd, c_int]),
+    "cblas_dtrmm": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int]),
+    "cblas_strmm": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_float, _pd, c_int, _pd, c_int]),
+    "dsyrk_": (None, [c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, POINTER(c_double), _pd, _pi]),
+    "ssyrk_": (None, [c_char_p, c_char_p, _pi, _pi, POINTER(c_float), _pd, _pi, POINTER(c_float), _pd, _pi]),
+    "dgemmt_": (None, [c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi, POINTER(c_double), _pd, _pi]),
+    "sgemmt_": (None, [c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi, POINTER(c_float), _pd, _pi]),
+    "dsymm_": (None, [c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi, POINTER(c_double), _pd, _pi]),
+    "ssymm_": (None, [c_char_p, c_char_p, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi, POINTER(c_float), _pd, _pi]),
+    "dtrmm_": (None, [c_char_p, c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi]),
+    "strmm_": (None, [c_char_p, c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi]),
     "dgetrf_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pi]),
     "sgetrf_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pi]),
     "dgetf2_": (c_int, [_pi, _pi, _pd, _pi, c_void_p, _pi]),

@@ -198,3 +198,52 @@ def dgemm_batch(As, Bs, Cs, alpha=1.0, beta=0.0):
     gs = np.ones(cnt, it)
     _fn("cblas_dgemm_batch")(I(102), _p(ta), _p(ta), _p(m), _p(n), _p(k), _p(al), pa, _p(m), pb, _p(k), _p(be), pc,
                              _p(m), I(cnt), _p(gs))
+
+
+def _ld(M, layout):
+    """Leading dimension of a 2-D array in the given CBLAS layout (its
+    memory must be Fortran-ordered for 102, C-ordered for 101)."""
+    return max(M.shape[0] if layout == 102 else M.shape[1], 1)
+
+
+def cblas_syrk(layout, uplo, trans, alpha, A, beta, C):
+    """uplo 121/122, trans 111/112; in place on C (n x n)."""
+    dt = C.dtype
+    I, F = _it(), (c_double if dt == np.float64 else c_float)
+    n = C.shape[0]
+    k = A.shape[0] if trans == 112 else A.shape[1]
+    _fn("cblas_dsyrk" if dt == np.float64 else "cblas_ssyrk")(I(layout), I(uplo), I(trans), I(n), I(k), F(alpha),
+                                                             _p(A), I(_ld(A, layout)), F(beta), _p(C),
+                                                             I(_ld(C, layout)))
+    return C
+
+
+def cblas_gemmt(layout, uplo, transa, transb, alpha, A, B, beta, C):
+    dt = C.dtype
+    I, F = _it(), (c_double if dt == np.float64 else c_float)
+    m = C.shape[0]
+    k = A.shape[0] if transa == 112 else A.shape[1]
+    _fn("cblas_dgemmt" if dt == np.float64 else "cblas_sgemmt")(I(layout), I(uplo), I(transa), I(transb), I(m), I(k),
+                                                               F(alpha), _p(A), I(_ld(A, layout)), _p(B),
+                                                               I(_ld(B, layout)), F(beta), _p(C), I(_ld(C, layout)))
+    return C
+
+
+def cblas_symm(layout, side, uplo, alpha, A, B, beta, C):
+    dt = C.dtype
+    I, F = _it(), (c_double if dt == np.float64 else c_float)
+    m, n = C.shape
+    _fn("cblas_dsymm" if dt == np.float64 else "cblas_ssymm")(I(layout), I(side), I(uplo), I(m), I(n), F(alpha), _p(A),
+                                                             I(_ld(A, layout)), _p(B), I(_ld(B, layout)), F(beta),
+                                                             _p(C), I(_ld(C, layout)))
+    return C
+
+
+def cblas_trmm(layout, side, uplo, transa, diag, alpha, A, B):
+    dt = B.dtype
+    I, F = _it(), (c_double if dt == np.float64 else c_float)
+    m, n = B.shape
+    _fn("cblas_dtrmm" if dt == np.float64 else "cblas_strmm")(I(layout), I(side), I(uplo), I(transa), I(diag), I(m),
+                                                             I(n), F(alpha), _p(A), I(_ld(A, layout)), _p(B),
+                                                             I(_ld(B, layout)))
+    return B

@@ -6,6 +6,7 @@ GPU tests check libopenblas_amd against them.
 
     python tests/golden/make_golden.py           # level3_golden.npz
     python tests/golden/make_golden.py --extra   # level3_golden_extra.npz
+    python tests/golden/make_golden.py --ext3    # level3_ext_golden.npz
 """
 import os
 import sys
@@ -201,8 +202,168 @@ def main_extra():
     print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
 
 
+# ---- SYRK / GEMMT / SYMM / TRMM ---------------------------------------------------
+# Inputs are multiples of 0.5 in [-2, 2] and alpha, beta dyadic, so every
+# product and sum is exact in FP32 and FP64: the expected outputs are exact
+# and a correct implementation must reproduce them bit for bit.  The
+# reference is called in both CBLAS layouts on the same logical matrices
+# (row-major: C-ordered arrays) and must give the same logical result.
+X3_N, X3_K, X3_SN, X3_TN = 66, 20, 45, 70
+X3_AB = ((0.75, 1.25), (1.5, 0.0))   # the second: beta = 0 with NaN in C
+
+
+def x3_inputs(rng):
+    base = lambda: (rng.integers(-4, 5, (X3_TN, X3_TN)) / 2.0).astype(np.float32)
+    return base(), base(), base()
+
+
+def x3_poison_tri(C, lower, also_ref):
+    """NaN in the triangle of C that the update must not touch (strictly
+    above for lower, below for upper); with also_ref, in the updated
+    triangle as well (beta = 0 must overwrite it)."""
+    C = C.copy()
+    n = C.shape[0]
+    other = np.triu(np.ones((n, n), bool), 1) if lower else np.tril(np.ones((n, n), bool), -1)
+    C[other] = np.nan
+    if also_ref:
+        C[~other] = np.nan
+    return C
+
+
+def x3_poison_a(A, lower, unit):
+    """NaN in the unreferenced strict triangle of A (and its diagonal when
+    unit)."""
+    A = A.copy()
+    n = A.shape[0]
+    other = np.triu(np.ones((n, n), bool), 1) if lower else np.tril(np.ones((n, n), bool), -1)
+    A[other] = np.nan
+    if unit:
+        A[np.eye(n, dtype=bool)] = np.nan
+    return A
+
+
+def x3_cases():
+    """(routine, params) of every fixture; params are the CBLAS enums."""
+    cases = []
+    for uplo in (121, 122):
+        for tr in (111, 112):
+            for ab in range(2):
+                cases.append(("syrk", (uplo, tr, ab)))
+    for uplo in (121, 122):
+        for ta in (111, 112):
+            for tb in (111, 112):
+                for ab in range(2):
+                    cases.append(("gemmt", (uplo, ta, tb, ab)))
+    for side in (141, 142):
+        for uplo in (121, 122):
+            for ab in range(2):
+                cases.append(("symm", (side, uplo, ab)))
+    for side in (141, 142):
+        for uplo in (121, 122):
+            for tr in (111, 112):
+                for diag in (131, 132):
+                    cases.append(("trmm", (side, uplo, tr, diag, 0)))
+    cases.append(("trmm", (141, 122, 111, 131, 1)))   # alpha = 0: B <- 0
+    return cases
+
+
+def x3_operands(routine, params, A0, B0, C0, dt):
+    """Logical input matrices of one case (float arrays of dtype dt)."""
+    n, k = X3_N, X3_K
+    if routine == "syrk":
+        uplo, tr, ab = params
+        A = A0[:k, :n] if tr == 112 else A0[:n, :k]
+        C = x3_poison_tri(C0[:n, :n], uplo == 122, ab == 1)
+        return A.astype(dt), None, C.astype(dt)
+    if routine == "gemmt":
+        uplo, ta, tb, ab = params
+        A = A0[:k, :n] if ta == 112 else A0[:n, :k]
+        B = B0[:n, :k] if tb == 112 else B0[:k, :n]
+        C = x3_poison_tri(C0[:n, :n], uplo == 122, ab == 1)
+        return A.astype(dt), B.astype(dt), C.astype(dt)
+    if routine == "symm":
+        side, uplo, ab = params
+        m, nn = n, X3_SN
+        na = nn if side == 142 else m
+        A = x3_poison_a(A0[:na, :na], uplo == 122, False)
+        C = C0[:m, :nn].copy()
+        if ab == 1:
+            C[:] = np.nan
+        return A.astype(dt), B0[:m, :nn].astype(dt), C.astype(dt)
+    side, uplo, tr, diag, az = params
+    m, nn = n, X3_TN
+    na = nn if side == 142 else m
+    A = x3_poison_a(A0[:na, :na], uplo == 122, diag == 132)
+    return A.astype(dt), B0[:m, :nn].astype(dt), None
+
+
+def x3_alpha_beta(routine, params):
+    if routine == "trmm":
+        return (0.0 if params[-1] else 0.75), None
+    return X3_AB[params[-1]]
+
+
+def x3_key(routine, params, tag):
+    return f"{routine}_{tag}_" + "_".join(str(p) for p in params)
+
+
+def x3_call(routine, params, layout, alpha, beta, A, B, C):
+    """The reference on logical matrices in one layout; returns the logical
+    result."""
+    conv = (lambda X: np.asfortranarray(X)) if layout == 102 else (lambda X: np.ascontiguousarray(X))
+    A = conv(A)
+    if B is not None:
+        B = conv(B)
+    if C is not None:
+        C = conv(C)
+    if routine == "syrk":
+        return R.cblas_syrk(layout, params[0], params[1], alpha, A, beta, C)
+    if routine == "gemmt":
+        return R.cblas_gemmt(layout, params[0], params[1], params[2], alpha, A, B, beta, C)
+    if routine == "symm":
+        return R.cblas_symm(layout, params[0], params[1], alpha, A, B, beta, C)
+    return R.cblas_trmm(layout, params[0], params[1], params[2], params[3], alpha, A, B)
+
+
+def main_ext3():
+    """tests/golden/level3_ext_golden.npz: SYRK, GEMMT, SYMM, TRMM (every
+    uplo/trans/side/diag, beta = 0 with NaN in C, NaN in the parts of A and C
+    that must not be read or written), FP64 and FP32, exact outputs."""
+    rng = np.random.default_rng(20261204)
+    A0, B0, C0 = x3_inputs(rng)
+    out = {"x3_A": A0, "x3_B": B0, "x3_C": C0}
+    for dt, tag in ((np.float64, "d"), (np.float32, "s")):
+        for routine, params in x3_cases():
+            A, B, C = x3_operands(routine, params, A0, B0, C0, dt)
+            alpha, beta = x3_alpha_beta(routine, params)
+            res = []
+            for layout in (102, 101):
+                Cin = None if C is None else C.copy()
+                Bin = None if B is None else B.copy()
+                res.append(np.array(x3_call(routine, params, layout, alpha, beta, A, Bin, Cin)))
+            if routine == "gemmt":
+                # The installed 0.3.28 build's row-major cblas_?gemmt does not
+                # give the triangle of op(A) op(B) that the reference's own
+                # utest/test_extensions/test_dgemmt.c requires (it compares with
+                # cblas_dgemm); the column-major call of the same logical
+                # problem does, and is the fixture.  The mismatch is recorded.
+                out.setdefault("gemmt_rowmajor_mismatch", [])
+                if not np.array_equal(res[0], res[1], equal_nan=True):
+                    out["gemmt_rowmajor_mismatch"].append(x3_key(routine, params, tag))
+            else:
+                assert np.array_equal(res[0], res[1], equal_nan=True), (routine, params, tag)
+            out[x3_key(routine, params, tag)] = res[0].astype(np.float32)   # exact: small dyadic values
+    out["gemmt_rowmajor_mismatch"] = np.array(out.get("gemmt_rowmajor_mismatch", []), dtype=str)
+    out["reference_config"] = np.array(R.config())
+    path = os.path.join(HERE, "level3_ext_golden.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes,", len(out), "arrays")
+
+
 if __name__ == "__main__":
     if "--extra" in sys.argv:
         main_extra()
+    elif "--ext3" in sys.argv:
+        main_ext3()
     else:
         main()

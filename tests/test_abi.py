@@ -51,7 +51,10 @@ def test_reference_symbol_set_covered():
            "dgemm_", "sgemm_", "dtrsm_", "strsm_", "dgetrf_", "sgetrf_", "dgetrs_", "sgetrs_", "dgesv_", "sgesv_",
            "dpotrf_", "spotrf_", "dlaswp_", "slaswp_", "dgetf2_", "sgetf2_", "dpotf2_", "spotf2_", "dsgesv_",
            "LAPACKE_dgesv", "LAPACKE_dsgesv", "LAPACKE_dgetrf", "LAPACKE_dgetrs", "LAPACKE_dpotrf", "xerbla_",
-           "openblas_get_config", "openblas_set_num_threads"]
+           "openblas_get_config", "openblas_set_num_threads",
+           "cblas_dsyrk", "cblas_ssyrk", "cblas_dsymm", "cblas_ssymm", "cblas_dtrmm", "cblas_strmm",
+           "cblas_dgemmt", "cblas_sgemmt", "dsyrk_", "ssyrk_", "dsymm_", "ssymm_", "dtrmm_", "strmm_",
+           "dgemmt_", "sgemmt_"]
     declared = set(header_functions())
     assert not [r for r in ref if r not in declared]
 
@@ -198,3 +201,30 @@ def test_batched_argument_errors_without_device(L):
     assert L.openblas_amd_dpotrf_batch(b"L", pi(i32([6])), nul, pi(i32([5])), pi(info), 1) == 0
     assert list(info) == [-4]
     assert L.openblas_amd_spotrf_batch(b"Q", pi(info), nul, pi(info), pi(info), 1) == -1
+
+
+def test_ext3_argument_errors(L):
+    """xerbla numbers of interface/syrk.c, gemmt.c, symm.c, trsm.c (TRMM)."""
+    z = np.zeros(4)
+    oa.last_xerbla()
+    L.cblas_dsyrk(102, 121, 111, 4, 2, 1.0, z.ctypes.data, 3, 0.0, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DSYRK", 7)
+    L.cblas_dsyrk(102, 121, 111, 4, 2, 1.0, z.ctypes.data, 4, 0.0, z.ctypes.data, 3)
+    assert oa.last_xerbla() == ("DSYRK", 10)
+    L.cblas_dsyrk(102, 99, 111, 4, 2, 1.0, z.ctypes.data, 4, 0.0, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DSYRK", 1)
+    L.cblas_dgemmt(102, 121, 111, 111, 4, 2, 1.0, z.ctypes.data, 4, z.ctypes.data, 1, 0.0, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DGEMMT", 10)
+    L.cblas_dgemmt(101, 121, 111, 111, 4, 2, 1.0, z.ctypes.data, 1, z.ctypes.data, 4, 0.0, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DGEMMT", 8)
+    L.cblas_dsymm(102, 142, 121, 4, 5, 1.0, z.ctypes.data, 4, z.ctypes.data, 4, 0.0, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DSYMM", 7)
+    L.cblas_dtrmm(102, 141, 121, 111, 131, 4, 2, 1.0, z.ctypes.data, 3, z.ctypes.data, 4)
+    assert oa.last_xerbla() == ("DTRMM", 9)
+    L.dtrmm_(b"L", b"U", b"N", b"X", oa.ip(4), oa.ip(2), ctypes.byref(ctypes.c_double(1.0)), z.ctypes.data, oa.ip(4),
+             z.ctypes.data, oa.ip(4))
+    assert oa.last_xerbla() == ("DTRMM", 4)
+    # quick returns: n == 0 touches nothing
+    C = np.full((2, 2), 7.0)
+    L.cblas_dsyrk(102, 121, 111, 0, 2, 1.0, z.ctypes.data, 1, 0.0, C.ctypes.data, 1)
+    assert np.all(C == 7.0) and oa.last_xerbla() == ("", 0)
