@@ -706,7 +706,7 @@ def main():
 
     flops_solve = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
     dom = dominant(step_k)
-    roof = roofline_of(step_k, dom, "lu_panel2_16384_fp32.md" if dom == "lu_panel" else None) if dom else None
+    roof = roofline_of(step_k, dom, "lu_panel_16384_fp32.md" if dom == "lu_panel" else None) if dom else None
     out = {
         "metric": "dgesv N=16384 time to FP64 backward error: FP32-LU+IR vs FP64 LU; DGEMM TF/s",
         "value": 1000.0 / ir_med,

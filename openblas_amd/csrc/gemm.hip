@@ -48,11 +48,18 @@ template <> struct Vec16<float> { typedef float4 type; static constexpr int n = 
 // Loads one BK-deep slab of an operand tile (TILE along m or n) into
 // registers.  KC: the operand is contiguous along k in memory (Trans A /
 // NoTrans B).  VEC: 16-byte vector loads are legal (aligned base and ld).
+// The slab is staged k-major in LDS, s[BK][TILE + PAD] (PAD: 128 bytes of
+// doubles, 64 of floats, so that the four 16-lane rows of one fragment read
+// do not start in the same bank).
 template <typename T, int TILE, int BK, bool KC, bool VEC>
 struct TileLoader {
   static constexpr int V = VEC ? Vec16<T>::n : 1;
   static constexpr int NV = TILE * BK / V / 256;  // vectors per thread
   static_assert(NV * V * 256 == TILE * BK, "tile must divide over 256 threads");
+  static constexpr int PAD = 64 / sizeof(T) * (sizeof(T) == 8 ? 2 : 1);  // 16 doubles / 16 floats
+  static constexpr int LD = TILE + PAD;
+  static constexpr int SIZE = BK * LD;   // elements per buffer
+  __device__ static inline int at(int k, int mn) { return k * LD + mn; }
   T r[NV * V];
 
   // GUARD: per-element bounds checks (edge tiles, K tail); otherwise the
@@ -60,7 +67,7 @@ struct TileLoader {
   // unconditional 16-byte vector loads.
   template <bool GUARD>
   __device__ inline void load(const T* __restrict__ g, long ld, long mn0, long k0, long MN, long K) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & 255;   // thread within its 256-thread group
     constexpr bool full = !GUARD;
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -89,10 +96,9 @@ struct TileLoader {
       }
     }
   }
-  // writes the slab into a k-major LDS tile s[BK][TILE + PAD]
-  template <int LDT>
+  // writes the slab into its LDS buffer (layout above)
   __device__ inline void store(T* s) const {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & 255;
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       const int v = tid + i * 256;
@@ -104,21 +110,27 @@ struct TileLoader {
           T* xs = (T*)&x;
 #pragma unroll
           for (int e = 0; e < V; e++) xs[e] = r[i * V + e];
-          *(vt*)(s + k * LDT + mn) = x;
+          *(vt*)(s + at(k, mn)) = x;
         } else {
-          s[k * LDT + mn] = r[i];
+          s[at(k, mn)] = r[i];
         }
       } else {
         int mn = v / (BK / V), k = (v % (BK / V)) * V;
 #pragma unroll
-        for (int e = 0; e < V; e++) s[(k + e) * LDT + mn] = r[i * V + e];
+        for (int e = 0; e < V; e++) s[at(k + e, mn)] = r[i * V + e];
       }
     }
   }
 };
 
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool VEC>
-__global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T alpha,
+// KS > 1 (sliced K): the workgroup is KS groups of 4 wavefronts; group g
+// multiplies the g-th of KS equal, BK-aligned k ranges into its own
+// accumulators, and the KS partial tiles are summed in LDS in the fixed
+// order 0..KS-1 before the epilogue (one launch, no workspace, the same
+// result on every run).  It fills a CU with one workgroup when there are
+// only about as many tiles as CUs.
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool VEC, int KS = 1>
+__global__ __launch_bounds__(256 * KS, KS == 1 ? 2 : 1) void gemm_kernel(long M, long N, long K, T alpha,
                                                       const T* __restrict__ A, long lda,
                                                       const T* __restrict__ B, long ldb, T beta,
                                                       T* __restrict__ C, long ldc, int tiles_m,
@@ -133,17 +145,29 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
     K = min(kc, K - y * kc);
     C += y * cstride;
   }
-  constexpr int PAD = 64 / sizeof(T) * (sizeof(T) == 8 ? 2 : 1);  // 16 doubles / 16 floats
-  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  typedef TileLoader<T, BM, BK, TA, VEC> LA;
+  typedef TileLoader<T, BN, BK, !TB, VEC> LB;
   constexpr int WTM = BM / 2, WTN = BN / 2;  // 2x2 waves
   constexpr int FM = WTM / 16, FN = WTN / 16;
   typedef MF<T> mf;
   typedef typename mf::acc acc_t;
 
-  __shared__ T smem[2 * BK * (LDA_S + LDB_S)];
-  // buffer b of the A slab at smem + b*BK*LDA_S, of the B slab after both A's
-#define OBA_AS(b) (smem + (b) * BK * LDA_S)
-#define OBA_BS(b) (smem + 2 * BK * LDA_S + (b) * BK * LDB_S)
+  constexpr int STAGE = 2 * (LA::SIZE + LB::SIZE);   // one group's double-buffered slabs
+  constexpr int SMEM = KS == 1 ? STAGE : (KS * STAGE > KS * BM * BN ? KS * STAGE : KS * BM * BN);
+  __shared__ T smem[SMEM];
+  const int grp = KS == 1 ? 0 : (int)(threadIdx.x >> 8);
+  // buffer b of the A slab at st + b*LA::SIZE, of the B slab after both A's
+  T* const st = smem + grp * STAGE;
+#define OBA_AS(b) (st + (b) * LA::SIZE)
+#define OBA_BS(b) (st + 2 * LA::SIZE + (b) * LB::SIZE)
+  long kg = 0;   // sliced K: the groups' common k-range length, a multiple of BK
+  const long k_total = K;
+  if (KS > 1) {
+    kg = ((K + KS - 1) / KS + BK - 1) / BK * BK;
+    A += grp * kg * (TA ? 1 : lda);
+    B += grp * kg * (TB ? ldb : 1);
+    K = max(0L, min(kg, K - grp * kg));
+  }
 
   // XCD-aware tile order: workgroup b runs on XCD b%8; give each XCD a
   // contiguous range of the grouped tile order.
@@ -164,11 +188,11 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
   if (abort_flag && *abort_flag) return;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = (threadIdx.x >> 6) & 3;
   const int wm0 = (wave & 1) * WTM, wn0 = (wave >> 1) * WTN;
 
-  TileLoader<T, BM, BK, TA, VEC> la;
-  TileLoader<T, BN, BK, !TB, VEC> lb;
+  LA la;
+  LB lb;
 
   acc_t acc[FM][FN];
 #pragma unroll
@@ -185,9 +209,9 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
       const int kr = kk * 4 + (lane >> 4);
       T fa[FM], fb[FN];
 #pragma unroll
-      for (int i = 0; i < FM; i++) fa[i] = as[kr * LDA_S + wm0 + i * 16 + (lane & 15)];
+      for (int i = 0; i < FM; i++) fa[i] = as[LA::at(kr, wm0 + i * 16 + (lane & 15))];
 #pragma unroll
-      for (int j = 0; j < FN; j++) fb[j] = bs[kr * LDB_S + wn0 + j * 16 + (lane & 15)];
+      for (int j = 0; j < FN; j++) fb[j] = bs[LB::at(kr, wn0 + j * 16 + (lane & 15))];
 #pragma unroll
       for (int i = 0; i < FM; i++)
 #pragma unroll
@@ -197,31 +221,33 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
   // Full slabs run with unconditional loads on interior tiles (no control
   // flow around the prefetch, so its loads stay in flight across the MFMAs);
   // the K tail and edge tiles use guarded loads.
+  // With sliced K every group runs the same kg/BK slabs (the same barriers);
+  // a group whose range ends early loads zeros through the guarded loads.
   auto run = [&](auto guard_tag) __attribute__((always_inline)) {
     constexpr bool G = decltype(guard_tag)::value;
-    const int nkf = (int)(K / BK);
-    const bool tail = (K % BK) != 0;
+    const int nkf = KS > 1 ? (int)(kg / BK) : (int)(K / BK);
+    const bool tail = KS == 1 && (K % BK) != 0;
     if (nkf == 0) {
       la.template load<true>(A, lda, m0, 0, M, K);
       lb.template load<true>(B, ldb, n0, 0, N, K);
-      la.template store<LDA_S>(OBA_AS(0));
-      lb.template store<LDB_S>(OBA_BS(0));
+      la.store(OBA_AS(0));
+      lb.store(OBA_BS(0));
       __syncthreads();
       compute(0);
       return;
     }
     la.template load<G>(A, lda, m0, 0, M, K);
     lb.template load<G>(B, ldb, n0, 0, N, K);
-    la.template store<LDA_S>(OBA_AS(0));
-    lb.template store<LDB_S>(OBA_BS(0));
+    la.store(OBA_AS(0));
+    lb.store(OBA_BS(0));
     __syncthreads();
     for (int kt = 0; kt < nkf - 1; kt++) {
       const int cur = kt & 1;
       la.template load<G>(A, lda, m0, (long)(kt + 1) * BK, M, K);
       lb.template load<G>(B, ldb, n0, (long)(kt + 1) * BK, N, K);
       compute(cur);
-      la.template store<LDA_S>(OBA_AS(cur ^ 1));
-      lb.template store<LDB_S>(OBA_BS(cur ^ 1));
+      la.store(OBA_AS(cur ^ 1));
+      lb.store(OBA_BS(cur ^ 1));
       __syncthreads();
     }
     const int last = (nkf - 1) & 1;
@@ -231,19 +257,50 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(long M, long N, long K, T 
     }
     compute(last);
     if (tail) {
-      la.template store<LDA_S>(OBA_AS(last ^ 1));
-      lb.template store<LDB_S>(OBA_BS(last ^ 1));
+      la.store(OBA_AS(last ^ 1));
+      lb.store(OBA_BS(last ^ 1));
       __syncthreads();
       compute(last ^ 1);
     }
   };
-  const bool interior = (m0 + BM <= M) && (n0 + BN <= N);
+  // Unguarded loads on interior tiles; with sliced K also only when every
+  // group's range is full (K_total == KS*kg), so that the choice, and with
+  // it the barrier sequence, is the same for the whole workgroup.
+  const bool interior = (m0 + BM <= M) && (n0 + BN <= N) && (KS == 1 || k_total == KS * kg);
   if (interior) run(std::false_type{});
   else run(std::true_type{});
 
 #undef OBA_AS
 #undef OBA_BS
   const bool bz = (beta == T(0));
+  if constexpr (KS > 1) {
+    // partial tiles to LDS (column-major BM x BN per group), then every
+    // thread sums its elements over the groups in order and writes C
+    __syncthreads();
+    T* pt = smem + grp * BM * BN;
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+      for (int j = 0; j < FN; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          pt[(wn0 + j * 16 + mf::orow(lane, r)) * BM + wm0 + i * 16 + (lane & 15)] = acc[i][j][r];
+    __syncthreads();
+#pragma unroll
+    for (int e = threadIdx.x; e < BM * BN; e += 256 * KS) {
+      const long m = m0 + e % BM, n = n0 + e / BM;
+      T sum = smem[e];
+#pragma unroll
+      for (int g = 1; g < KS; g++) sum += smem[g * BM * BN + e];
+      if (m < M && n < N) {
+        T* p = C + m + n * ldc;
+        T v = alpha * sum;
+        if (!bz) v += beta * *p;
+        *p = v;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; i++) {
     const long m = m0 + wm0 + i * 16 + (lane & 15);
@@ -364,15 +421,15 @@ static void gemm_skinny(long m, long k, T alpha, const T* a, long lda, const T* 
   OBA_HIP(hipGetLastError());
 }
 
-template <typename T, int BM, int BN, int BK, bool VEC>
+template <typename T, int BM, int BN, int BK, bool VEC, int KS = 1>
 static void launch_cfg(bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
                        const T* B, long ldb, T beta, T* C, long ldc, hipStream_t s, int tri,
                        const int* abort_flag, int ksplit = 1, long kc = 0, long cstride = 0) {
   long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   if (tm * tn > 0x7fffffffL) fatal("gemm: grid too large", nullptr);
-  dim3 grid((unsigned)(tm * tn), (unsigned)ksplit), block(256);
+  dim3 grid((unsigned)(tm * tn), (unsigned)ksplit), block(256 * KS);
 #define OBA_GEMM_L(TA_, TB_) \
-  gemm_kernel<T, BM, BN, BK, TA_, TB_, VEC><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int)tm, (int)tn, tri, abort_flag, kc, cstride)
+  gemm_kernel<T, BM, BN, BK, TA_, TB_, VEC, KS><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int)tm, (int)tn, tri, abort_flag, kc, cstride)
   if (!ta && !tb) OBA_GEMM_L(false, false);
   else if (!ta && tb) OBA_GEMM_L(false, true);
   else if (ta && !tb) OBA_GEMM_L(true, false);
@@ -429,9 +486,20 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
   constexpr int BK_BIG = sizeof(T) == 8 ? 4 : 8;
   constexpr int BK_SMALL = sizeof(T) == 8 ? 8 : 16;
   const long tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
+  if (small && tri == 0 && abort_flag == nullptr && k >= 1024 && 8 * tiles64 >= 7 * rt().num_cu &&
+      tiles64 <= rt().num_cu) {
+    // about one 64x64 tile per CU, deep K: one workgroup of 4 wave groups per
+    // tile, each group a quarter of K, partials summed in LDS (4 wavefronts
+    // per SIMD instead of 1; no workspace, so any stream).  MI355X, 1024^3:
+    // DGEMM 68.7 -> 59.5 us, SGEMM 47.4 -> 40.3 us per call against the
+    // split across workgroups; with 192 tiles the split is still ahead.
+    if (vec) launch_cfg<T, 64, 64, BK_SMALL, true, 4>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, 0, nullptr);
+    else launch_cfg<T, 64, 64, BK_SMALL, false, 4>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, 0, nullptr);
+    return;
+  }
   if (ws_ok && small && tri == 0 && abort_flag == nullptr && k >= 1024 && tiles64 <= rt().num_cu) {
-    // few tiles, deep K: split K so that ~4 workgroups per CU hide each
-    // other's latencies (1024^3 DGEMM: one 64x64 tile per CU otherwise)
+    // fewer tiles than half the CUs, deep K: split K across workgroups so
+    // that ~4 of them per CU hide each other's latencies
     const int ks = (int)std::min<long>({8L, std::max<long>(1L, 4L * rt().num_cu / tiles64), k / 256});
     if (ks > 1) {
       const long kc = ((k + ks - 1) / ks + 15) / 16 * 16;

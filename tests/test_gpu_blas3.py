@@ -145,15 +145,19 @@ def test_gemm_few_columns(oba, dt):
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_gemm_split_k(oba, dt):
-    """Few tiles with a deep K take the split-K path (partial products in a
-    workspace, summed with beta*C by a second kernel): dmmch ratio for all
-    four transpose forms, odd leading dimensions and a K tail, beta == 0
-    over NaN."""
+    """The deep-K paths for few tiles: 300 x 250 x 3001 (20 tiles: K split
+    across workgroups, partials summed by a second kernel), 1000 x 1000 x
+    1100 (256 tiles: K sliced over the 4 wave groups of one workgroup, K not
+    a multiple of 4 slabs, so every group's range is guarded, M/N edges
+    inside a tile) and 1024^3 (sliced, unguarded); 65 x 1030 x 259 and
+    1 x 7 x 333 stay on the plain tile kernel.  dmmch ratio for all four
+    transpose forms, odd leading dimensions, rows of C beyond M untouched,
+    beta == 0 over NaN."""
     L = oba.lib()
     f = L.cblas_dgemm if dt == np.float64 else L.cblas_sgemm
     rng = np.random.default_rng(23)
-    m, n, k = 300, 250, 3001
-    for ta in (0, 1):
+    for (m, n, k) in ((300, 250, 3001), (1000, 1000, 1100), (1024, 1024, 1024), (65, 1030, 259), (1, 7, 333)):
+      for ta in (0, 1):
         for tb in (0, 1):
             A = np.asfortranarray(rng.uniform(-1, 1, (k + 1, m + 1) if ta else (m + 1, k)).astype(dt))
             B = np.asfortranarray(rng.uniform(-1, 1, (n + 1, k) if tb else (k + 1, n)).astype(dt))
@@ -163,13 +167,30 @@ def test_gemm_split_k(oba, dt):
               C.ctypes.data, m + 3)
             Av = A[:k, :m] if ta else A[:m, :k]
             Bv = B[:n, :k] if tb else B[:k, :n]
-            assert gemm_ratio(ta, tb, 0.9, Av, Bv, -0.4, C0[:m], C[:m], dt) < 16, (ta, tb)
+            assert gemm_ratio(ta, tb, 0.9, Av, Bv, -0.4, C0[:m], C[:m], dt) < 16, (m, n, k, ta, tb)
             assert np.array_equal(C[m:], C0[m:])   # rows beyond m untouched
     A = np.asfortranarray(rng.uniform(-1, 1, (1024, 1024)).astype(dt))
     C = np.full((1024, 1024), np.nan, dt, order="F")
     f(102, 111, 111, 1024, 1024, 1024, 1.0, A.ctypes.data, 1024, A.ctypes.data, 1024, 0.0, C.ctypes.data, 1024)
     assert not np.isnan(C).any()
     assert gemm_ratio(0, 0, 1.0, A, A, 0.0, np.zeros_like(C), C, dt) < 16
+    # device-resident operands with odd leading dimensions (scalar loads on
+    # the sliced path), twice: the fixed summation order gives equal bits
+    m, n, k = 1000, 999, 1500
+    A = np.asfortranarray(rng.uniform(-1, 1, (m + 1, k)).astype(dt))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k + 3, n)).astype(dt))
+    C0 = np.asfortranarray(rng.uniform(-1, 1, (m + 1, n)).astype(dt))
+    dA, dB = oba.DeviceBuffer.from_array(A), oba.DeviceBuffer.from_array(B)
+    outs = []
+    for _ in range(2):
+        dC = oba.DeviceBuffer.from_array(C0)
+        f(102, 111, 111, m, n, k, -1.5, dA.ptr, m + 1, dB.ptr, k + 3, 0.25, dC.ptr, m + 1)
+        outs.append(dC.to_array(np.empty_like(C0)))
+        dC.free()
+    dA.free(); dB.free()
+    assert np.array_equal(outs[0], outs[1])
+    assert gemm_ratio(0, 0, -1.5, A[:m], B[:k], 0.25, C0[:m], outs[0][:m], dt) < 16
+    assert np.array_equal(outs[0][m:], C0[m:])
 
 
 def test_fortran_gemm_matches_cblas(oba):
