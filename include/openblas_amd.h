@@ -73,6 +73,22 @@ void cblas_sgemm_batch(enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE *Trans
                        const float **A_array, const blasint *lda_array, const float **B_array,
                        const blasint *ldb_array, const float *beta_array, float **C_array,
                        const blasint *ldc_array, blasint group_count, const blasint *group_size);
+/* Strided-batch extension of the group API above (no counterpart in the
+   reference): batch_size problems of one shape, problem i at A + i*stridea,
+   B + i*strideb, C + i*stridec (element strides; 0 shares an operand).
+   Same argument checks as cblas_?gemm (numbers counted after Order:
+   lda 8, ldb 11, ldc 15), plus stridea < 0 -> 9, strideb < 0 -> 12,
+   stridec < 0 or overlapping C matrices -> 16, batch_size < 0 -> 17; xerbla
+   names DGEMM_STRIDED / SGEMM_STRIDED.  Runs on the same persistent batched
+   engine as cblas_?gemm_batch. */
+void cblas_dgemm_batch_strided(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                               blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
+                               long stridea, const double *B, blasint ldb, long strideb, double beta, double *C,
+                               blasint ldc, long stridec, blasint batch_size);
+void cblas_sgemm_batch_strided(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                               blasint M, blasint N, blasint K, float alpha, const float *A, blasint lda,
+                               long stridea, const float *B, blasint ldb, long strideb, float beta, float *C,
+                               blasint ldc, long stridec, blasint batch_size);
 
 /* cblas.h:313 (gemmt), :322 (symm), :331 (syrk), :349 (trmm); interface/syrk.c,
    symm.c, trsm.c (TRMM), gemmt.c built with -DCBLAS */

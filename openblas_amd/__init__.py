@@ -9,7 +9,7 @@ There is no CPU fallback: if the library is missing, ``lib()`` raises.
 """
 import ctypes
 import os
-from ctypes import POINTER, c_char, c_char_p, c_double, c_float, c_int, c_size_t, c_void_p
+from ctypes import POINTER, c_char, c_char_p, c_double, c_float, c_int, c_long, c_size_t, c_void_p
 
 import numpy as np
 
@@ -36,6 +36,10 @@ _SIGS = {
                                  POINTER(c_void_p), _pi, POINTER(c_double), POINTER(c_void_p), _pi, c_int, _pi]),
     "cblas_sgemm_batch": (None, [c_int, _pi, _pi, _pi, _pi, _pi, POINTER(c_float), POINTER(c_void_p), _pi,
                                  POINTER(c_void_p), _pi, POINTER(c_float), POINTER(c_void_p), _pi, c_int, _pi]),
+    "cblas_dgemm_batch_strided": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, c_long, _pd,
+                                         c_int, c_long, c_double, _pd, c_int, c_long, c_int]),
+    "cblas_sgemm_batch_strided": (None, [c_int, c_int, c_int, c_int, c_int, c_int, c_float, _pd, c_int, c_long, _pd,
+                                         c_int, c_long, c_float, _pd, c_int, c_long, c_int]),
     "dgemm_": (None, [c_char_p, c_char_p, _pi, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi, POINTER(c_double), _pd, _pi]),
     "sgemm_": (None, [c_char_p, c_char_p, _pi, _pi, _pi, POINTER(c_float), _pd, _pi, _pd, _pi, POINTER(c_float), _pd, _pi]),
     "dtrsm_": (None, [c_char_p, c_char_p, c_char_p, c_char_p, _pi, _pi, POINTER(c_double), _pd, _pi, _pd, _pi]),

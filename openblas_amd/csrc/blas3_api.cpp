@@ -243,6 +243,67 @@ void gemm_batch_impl(const char* name, int order, const int* ta_arr, const int* 
   gemm_batch_run<T>(probs);
 }
 
+// ---- GEMM batch, strided (extension of the group API) ----------------------------
+// One shape and one (alpha, beta) for batch_size problems; problem i takes
+// A + i*stridea, B + i*strideb, C + i*stridec (strides in elements).  The
+// single-GEMM checks of interface/gemm.c:423-471 keep their CBLAS numbers
+// (parameters counted after Order: TransA 1 ... lda 8, ldb 11, ldc 15);
+// negative strides (9, 12, 16), C matrices that would overlap (16) and a
+// negative batch_size (17) are errors too.  A or B strides of 0 share one
+// operand across the batch.  Row-major swaps A and B before the checks, and
+// the numbers are those of the swapped call, as in the group API
+// (interface/gemm_batch.c:248-290).
+template <typename T>
+void gemm_batch_strided_impl(const char* name, int order, int TransA, int TransB, blasint m, blasint n, blasint k,
+                             T alpha, const T* a, blasint lda, long stridea, const T* b, blasint ldb, long strideb,
+                             T beta, T* c, blasint ldc, long stridec, blasint batch) {
+  int transa = -1, transb = -1;
+  long M = 0, N = 0, K = k, LDA = 0, LDB = 0, SA = 0, SB = 0;
+  const T *pa = nullptr, *pb = nullptr;
+  blasint info = 0;
+  if (order == ColMajor) {
+    M = m; N = n; pa = a; pb = b; LDA = lda; LDB = ldb; SA = stridea; SB = strideb;
+    transa = cblas_trans(TransA); transb = cblas_trans(TransB);
+  } else if (order == RowMajor) {
+    M = n; N = m; pa = b; pb = a; LDA = ldb; LDB = lda; SA = strideb; SB = stridea;
+    transa = cblas_trans(TransB); transb = cblas_trans(TransA);
+  }
+  if (order == ColMajor || order == RowMajor) {
+    long nrowa = transa == 1 ? K : M;
+    long nrowb = transb == 1 ? N : K;
+    info = -1;
+    if (batch < 0) info = 17;
+    if (stridec < 0 || (batch > 1 && M > 0 && N > 0 && stridec < (long)ldc * N)) info = 16;
+    if (ldc < M) info = 15;
+    if (strideb < 0) info = 12;
+    if (LDB < nrowb) info = 11;
+    if (stridea < 0) info = 9;
+    if (LDA < nrowa) info = 8;
+    if (K < 0) info = 5;
+    if (N < 0) info = 4;
+    if (M < 0) info = 3;
+    if (transb < 0) info = 2;
+    if (transa < 0) info = 1;
+  }
+  if (info >= 0) { report_xerbla(name, info); return; }
+  if (M == 0 || N == 0 || batch == 0) return;
+  std::vector<BDesc> probs((size_t)batch);
+  for (blasint i = 0; i < batch; i++) {
+    BDesc& d = probs[i];
+    d = BDesc{};
+    d.a = (void*)(pa + i * SA);
+    d.b = (void*)(pb + i * SB);
+    d.c = (void*)(c + i * stridec);
+    d.m = (int)M; d.n = (int)N; d.k = (int)K;
+    d.lda = (int)LDA; d.ldb = (int)LDB; d.ldc = (int)ldc;
+    d.idx = (int)i;
+    d.flags = transa | (transb << 1);
+    d.alpha = (double)alpha;
+    d.beta = (double)beta;
+  }
+  gemm_batch_run<T>(probs);
+}
+
 }  // namespace
 
 // ---- exported symbols ----------------------------------------------------------
@@ -286,13 +347,31 @@ OBA_API void cblas_dgemm_batch(int order, const int* ta, const int* tb, const bl
                                const blasint* k, const double* alpha, const double** a, const blasint* lda,
                                const double** b, const blasint* ldb, const double* beta, double** c,
                                const blasint* ldc, blasint group_count, const blasint* group_size) {
-  guarded([&] { gemm_batch_impl<double>("DGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
+  guarded([&] { gemm_batch_impl<double>("DGEMM_BATCH ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
                           group_size); });
+}
+OBA_API void cblas_dgemm_batch_strided(int order, int ta, int tb, blasint m, blasint n, blasint k, double alpha,
+                                       const double* a, blasint lda, long stridea, const double* b, blasint ldb,
+                                       long strideb, double beta, double* c, blasint ldc, long stridec,
+                                       blasint batch_size) {
+  guarded([&] {
+    gemm_batch_strided_impl<double>("DGEMM_STRIDED ", order, ta, tb, m, n, k, alpha, a, lda, stridea, b, ldb,
+                                    strideb, beta, c, ldc, stridec, batch_size);
+  });
+}
+OBA_API void cblas_sgemm_batch_strided(int order, int ta, int tb, blasint m, blasint n, blasint k, float alpha,
+                                       const float* a, blasint lda, long stridea, const float* b, blasint ldb,
+                                       long strideb, float beta, float* c, blasint ldc, long stridec,
+                                       blasint batch_size) {
+  guarded([&] {
+    gemm_batch_strided_impl<float>("SGEMM_STRIDED ", order, ta, tb, m, n, k, alpha, a, lda, stridea, b, ldb,
+                                   strideb, beta, c, ldc, stridec, batch_size);
+  });
 }
 OBA_API void cblas_sgemm_batch(int order, const int* ta, const int* tb, const blasint* m, const blasint* n,
                                const blasint* k, const float* alpha, const float** a, const blasint* lda,
                                const float** b, const blasint* ldb, const float* beta, float** c,
                                const blasint* ldc, blasint group_count, const blasint* group_size) {
-  guarded([&] { gemm_batch_impl<float>("SGEMM ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
+  guarded([&] { gemm_batch_impl<float>("SGEMM_BATCH ", order, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_count,
                          group_size); });
 }

@@ -61,6 +61,8 @@ struct Runtime {
   hipStream_t side = nullptr;     // second stream (same priority)
   hipStream_t bulk = nullptr;     // like side, restricted to all but a few CUs
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  hipEvent_t ev_free = nullptr;   // pool_free's ordering event
+  bool bulk_used = false;         // bulk_stream() has been handed out
   bool sync_on_return = true;     // synchronous semantics like the reference
   // Sticky device error word in pinned, device-mapped host memory: kernels
   // with a bounded wait set it (system-scope store) when the wait runs out;
@@ -76,10 +78,13 @@ Runtime& rt();                    // initializes on first use
 hipStream_t bulk_stream();
 
 // Device scratch pool (the analogue of blas_memory_alloc,
-// driver/others/memory.c:1161): cached blocks, reused across calls.  A freed
-// block is handed out again at once, so pool memory may only be used by work
-// on rt().stream (stream order then separates the users); code that runs on
-// another stream must not allocate from the pool.
+// driver/others/memory.c:1161): cached blocks, reused across calls.  Blocks
+// are handed out for work on rt().stream, where stream order separates
+// successive users.  A block may also be used by work on the bulk stream
+// that waits on rt().stream: once bulk_stream() has been handed out, every
+// pool_free makes rt().stream wait for the bulk stream's work queued so far
+// before the block returns to the cache, so a later user on rt().stream runs
+// after it.
 void* pool_alloc(size_t bytes);
 void pool_free(void* p);
 

@@ -76,6 +76,7 @@ Runtime& rt() {
     OBA_HIP(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
     OBA_HIP(hipEventCreateWithFlags(&r->ev_a, hipEventDisableTiming));
     OBA_HIP(hipEventCreateWithFlags(&r->ev_b, hipEventDisableTiming));
+    OBA_HIP(hipEventCreateWithFlags(&r->ev_free, hipEventDisableTiming));
     void* he = nullptr;
     OBA_HIP(hipHostMalloc(&he, 256, hipHostMallocMapped));
     void* de = nullptr;
@@ -97,6 +98,7 @@ Runtime& rt() {
 
 hipStream_t bulk_stream() {
   Runtime& r = rt();
+  r.bulk_used = true;   // from now on pool_free orders reuse after the bulk stream's work
   static std::once_flag once;
   std::call_once(once, [&r] {
     int reserve = 32;
@@ -255,6 +257,15 @@ void* pool_alloc(size_t bytes) {
 
 void pool_free(void* p) {
   if (!p) return;
+  Runtime& r = *g_rt;
+  if (r.bulk_used) {
+    // the block may have been used on the bulk stream: everything queued
+    // later on the library stream (where the block is handed out again)
+    // waits for the bulk stream's work so far.  Errors are left for the next
+    // checked call (this runs in destructors).
+    if (hipEventRecord(r.ev_free, r.bulk) != hipSuccess || hipStreamWaitEvent(r.stream, r.ev_free, 0) != hipSuccess)
+      (void)hipGetLastError();
+  }
   std::lock_guard<std::mutex> g(g_pool_mu);
   auto it = g_size.find(p);
   if (it == g_size.end()) return;   // not a pool block (cannot happen; never throw from a destructor)
@@ -302,19 +313,26 @@ static thread_local char g_last_name[16] = {0};
 static thread_local int g_last_info = 0;
 
 typedef int (*xerbla_fn)(char*, blasint*, blasint);
+}  // namespace oba
+extern "C" int xerbla_(char* message, blasint* info, blasint len);
+namespace oba {
 
 void report_xerbla(const char* name, blasint info) {
   strncpy(g_last_name, name, sizeof(g_last_name) - 1);
   g_last_info = info;
-  // Route through whichever xerbla_ wins symbol resolution in the process
-  // (a user override, as the reference's weak alias allows), else ours.
+  // Route through the xerbla_ that the program itself resolves (a user
+  // override, as the reference's weak alias allows), else ours.  The lookup
+  // goes through the main program's handle: this library is linked with
+  // -Bsymbolic, and for such an object dlsym(RTLD_DEFAULT, ...) searches the
+  // object itself first and would always return its own definition.
   char buf[16];
   strncpy(buf, name, sizeof(buf) - 1);
   buf[sizeof(buf) - 1] = 0;
-  xerbla_fn f = (xerbla_fn)dlsym(RTLD_DEFAULT, "xerbla_");
+  static void* const prog = dlopen(nullptr, RTLD_LAZY);
+  xerbla_fn f = prog ? (xerbla_fn)dlsym(prog, "xerbla_") : nullptr;
+  if (!f) f = &xerbla_;
   blasint i = info;
-  if (f) f(buf, &i, (blasint)strlen(buf));
-  else printf(" ** On entry to %6s parameter number %2d had an illegal value\n", buf, i);
+  f(buf, &i, (blasint)strlen(buf));
 }
 
 }  // namespace oba

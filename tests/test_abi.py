@@ -228,3 +228,45 @@ def test_ext3_argument_errors(L):
     C = np.full((2, 2), 7.0)
     L.cblas_dsyrk(102, 121, 111, 0, 2, 1.0, z.ctypes.data, 1, 0.0, C.ctypes.data, 1)
     assert np.all(C == 7.0) and oa.last_xerbla() == ("", 0)
+
+
+def test_gemm_batch_strided_argument_errors(L):
+    """xerbla numbers of the strided-batch extension (checks of interface/gemm.c
+    numbered after Order, plus the strides and the batch size); a zero batch,
+    M or N touches nothing.  Nothing here reaches the device."""
+    z = np.zeros(64)
+    C = np.full(8, 7.0)
+    f = L.cblas_dgemm_batch_strided
+    cases = [
+        ((102, 99, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 1),
+        ((102, 111, 99, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 2),
+        ((102, 111, 111, -1, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 3),
+        ((102, 111, 111, 2, -1, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 4),
+        ((102, 111, 111, 2, 2, -1, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 5),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 1, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 8),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, -4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 2), 9),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 1, 4, 0.0, C.ctypes.data, 2, 4, 2), 11),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, -1, 0.0, C.ctypes.data, 2, 4, 2), 12),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 1, 4, 2), 15),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 3, 2), 16),
+        ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, -1), 17),
+        # row-major: A and B swap before the checks, so the row-major lda is checked as ldb (11)
+        ((101, 111, 111, 2, 3, 2, 1.0, z.ctypes.data, 1, 4, z.ctypes.data, 3, 6, 0.0, C.ctypes.data, 3, 6, 2), 11),
+        ((101, 111, 111, 2, 3, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 6, 0.0, C.ctypes.data, 3, 6, 2), 8),
+    ]
+    for args, exp in cases:
+        oa.last_xerbla()
+        f(*args)
+        assert oa.last_xerbla() == ("DGEMM_STRIDED", exp), (args, exp)
+        assert (C == 7.0).all()
+    oa.last_xerbla()
+    L.cblas_sgemm_batch_strided(102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0,
+                                C.ctypes.data, 2, 3, 2)
+    assert oa.last_xerbla() == ("SGEMM_STRIDED", 16)
+    # quick returns: batch 0, M 0, N 0 (a single problem may use stride 0 for C)
+    for args in ((102, 111, 111, 2, 2, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 4, 0),
+                 (102, 111, 111, 0, 2, 2, 1.0, z.ctypes.data, 1, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 1, 4, 3),
+                 (102, 111, 111, 2, 0, 2, 1.0, z.ctypes.data, 2, 4, z.ctypes.data, 2, 4, 0.0, C.ctypes.data, 2, 0, 3)):
+        oa.last_xerbla()
+        f(*args)
+        assert oa.last_xerbla() == ("", 0) and (C == 7.0).all()

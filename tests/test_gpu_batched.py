@@ -3,7 +3,8 @@
 * cblas_?gemm_batch (reference interface/gemm_batch.c:117): every problem
   equals the oracle GEMM (dmmch ratio), all four transpose pairs, uneven
   sizes, alpha/beta edge values, NaN in C with beta == 0, host and device
-  operands, row-major groups;
+  operands, row-major groups; the strided-batch extension bit-for-bit
+  against the group API on the same problems;
 * openblas_amd_?getrf_batch / ?getrs_batch / ?potrf_batch (no reference
   batched routine): every problem gives the single-problem oracle's result
   (getf2 ipiv and info bit-exact on tie-free data, factors within c*n*eps,
@@ -134,8 +135,89 @@ def test_gemm_batch_row_major_and_errors(oba):
                         pi(iarr([k, k])), parr([B.ctypes.data] * 2), pi(iarr([n, n - 1])),
                         (ctypes.c_double * 2)(0.0, 0.0), parr([C2.ctypes.data] * 2), pi(iarr([n, n])), 2,
                         pi(iarr([1, 1])))
-    assert oba.last_xerbla() == ("DGEMM", 8)   # row-major: ldb of the row-major call checks as lda
+    assert oba.last_xerbla() == ("DGEMM_BATCH", 8)   # row-major: ldb of the row-major call checks as lda
     assert (C2 == 7.0).all()
+
+
+@pytest.mark.parametrize("tag", ["d", "s"])
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_gemm_batch_strided(oba, tag, where):
+    """cblas_?gemm_batch_strided (extension): for every transpose pair and
+    both layouts, 40 problems at strides with gaps between the matrices
+    (and a B stride of 0, one operand shared by the batch) give exactly the
+    bits of the group API called with the same pointers (both run the same
+    descriptors through the same engine), and each problem passes the dmmch
+    ratio against the FP64 product; beta == 0 over NaN; the gaps between the
+    C matrices are not written."""
+    L = oba.lib()
+    dt = np.float64 if tag == "d" else np.float32
+    fs = L.cblas_dgemm_batch_strided if tag == "d" else L.cblas_sgemm_batch_strided
+    fg = L.cblas_dgemm_batch if tag == "d" else L.cblas_sgemm_batch
+    cty = ctypes.c_double if tag == "d" else ctypes.c_float
+    rng = np.random.default_rng(71)
+    cnt, m, n, k = 40, 37, 29, 45
+    for order in (102, 101):
+        for ta in (111, 112):
+            for tb in (111, 112):
+                # logical op(A) m x k, op(B) k x n; stored per the layout
+                ra, ca = (k, m) if ta == 112 else (m, k)
+                rb, cb = (n, k) if tb == 112 else (k, n)
+                if order == 101:
+                    ra, ca, rb, cb, rc, cc = ca, ra, cb, rb, n, m   # column-major view of the row-major data
+                else:
+                    rc, cc = m, n
+                lda, ldb, ldc = ra + 3, rb + 1, rc + 2
+                sa, sb, sc = lda * ca + 5, 0 if ta == tb else ldb * cb + 7, ldc * cc + 11
+                beta = 0.0 if ta == 111 else -0.5
+                A = rng.uniform(-1, 1, sa * (cnt - 1) + lda * ca).astype(dt)
+                B = rng.uniform(-1, 1, max(sb * (cnt - 1), 0) + ldb * cb).astype(dt)
+                C0 = rng.uniform(-1, 1, sc * (cnt - 1) + ldc * cc).astype(dt)
+                if beta == 0.0:
+                    C0[::3] = np.nan
+                outs = []
+                for api in ("strided", "group"):
+                    C = C0.copy()
+                    if where == "device":
+                        pool = DevPool(oba, [A, B, C])
+                        pa, pb, pc = pool.ptr(0), pool.ptr(1), pool.ptr(2)
+                    else:
+                        pa, pb, pc = A.ctypes.data, B.ctypes.data, C.ctypes.data
+                    es = A.itemsize
+                    if api == "strided":
+                        fs(order, ta, tb, m, n, k, 0.75, pa, lda, sa, pb, ldb, sb, beta, pc, ldc, sc, cnt)
+                    else:
+                        fg(order, pi(iarr([ta])), pi(iarr([tb])), pi(iarr([m])), pi(iarr([n])), pi(iarr([k])),
+                           (cty * 1)(0.75), parr([pa + i * sa * es for i in range(cnt)]), pi(iarr([lda])),
+                           parr([pb + i * sb * es for i in range(cnt)]), pi(iarr([ldb])), (cty * 1)(beta),
+                           parr([pc + i * sc * es for i in range(cnt)]), pi(iarr([ldc])), 1, pi(iarr([cnt])))
+                    if where == "device":
+                        pool.get(2, C)
+                        pool.buf.free()
+                    outs.append(C)
+                assert np.array_equal(outs[0], outs[1], equal_nan=True), (order, ta, tb)
+                C = outs[0]
+                for i in range(cnt):
+                    def mat(X, off, ld, r, c):
+                        return X[off:off + ld * c].reshape(c, ld).T[:r]
+                    Ai = mat(A, i * sa, lda, ra, ca).astype(np.float64)
+                    Bi = mat(B, i * sb, ldb, rb, cb).astype(np.float64)
+                    Ci0 = np.nan_to_num(mat(C0, i * sc, ldc, rc, cc).astype(np.float64)) if beta != 0.0 else 0.0
+                    Ci = mat(C, i * sc, ldc, rc, cc).astype(np.float64)
+                    if order == 101:   # the column-major views hold C^T = op(B)^T op(A)^T, A^T, B^T
+                        oA = Bi if tb == 111 else Bi.T
+                        oB = Ai if ta == 111 else Ai.T
+                    else:
+                        oA = Ai.T if ta == 112 else Ai
+                        oB = Bi.T if tb == 112 else Bi
+                    ref = 0.75 * oA @ oB + beta * Ci0
+                    g = 0.75 * np.abs(oA) @ np.abs(oB) + abs(beta) * np.abs(Ci0)
+                    assert np.max(np.abs(Ci - ref) / g) / EPS[dt] < 16, (order, ta, tb, i)
+                    # the gap after C_i (rows ldc-rc of each column, and the stride slack) keeps C0's bits
+                    end = i * sc + ldc * cc
+                    full = C[i * sc:end].reshape(cc, ldc)[:, rc:]
+                    assert np.array_equal(full, C0[i * sc:end].reshape(cc, ldc)[:, rc:], equal_nan=True)
+                    if i < cnt - 1:
+                        assert np.array_equal(C[end:(i + 1) * sc], C0[end:(i + 1) * sc], equal_nan=True)
 
 
 # ---- batched LU ----------------------------------------------------------------
