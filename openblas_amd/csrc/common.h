@@ -62,7 +62,6 @@ struct Runtime {
   hipStream_t bulk = nullptr;     // like side, restricted to all but a few CUs
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   hipEvent_t ev_free = nullptr;   // pool_free's ordering event
-  bool bulk_used = false;         // bulk_stream() has been handed out
   bool sync_on_return = true;     // synchronous semantics like the reference
   // Sticky device error word in pinned, device-mapped host memory: kernels
   // with a bounded wait set it (system-scope store) when the wait runs out;
@@ -80,24 +79,29 @@ hipStream_t bulk_stream();
 // Device scratch pool (the analogue of blas_memory_alloc,
 // driver/others/memory.c:1161): cached blocks, reused across calls.  Blocks
 // are handed out for work on rt().stream, where stream order separates
-// successive users.  A block may also be used by work on the bulk stream
-// that waits on rt().stream: once bulk_stream() has been handed out, every
-// pool_free makes rt().stream wait for the bulk stream's work queued so far
-// before the block returns to the cache, so a later user on rt().stream runs
-// after it.
+// successive users.  A block that work on another stream also uses is
+// marked (DevBuf::use_on); freeing it makes rt().stream wait for that
+// stream's work queued so far, so the next user of the block, on
+// rt().stream, runs after it.
 void* pool_alloc(size_t bytes);
-void pool_free(void* p);
+void pool_free(void* p, hipStream_t used_on = nullptr);
 
 struct DevBuf {
   void* p = nullptr;
+  hipStream_t other = nullptr;   // another stream that uses the block
   DevBuf() = default;
   explicit DevBuf(size_t bytes) { if (bytes) p = pool_alloc(bytes); }
-  ~DevBuf() { if (p) pool_free(p); }
+  ~DevBuf() { if (p) pool_free(p, other); }
+  void use_on(hipStream_t s) { other = s; }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), other(o.other) { o.p = nullptr; o.other = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { if (p) pool_free(p); p = o.p; o.p = nullptr; }
+    if (this != &o) {
+      if (p) pool_free(p, other);
+      p = o.p; other = o.other;
+      o.p = nullptr; o.other = nullptr;
+    }
     return *this;
   }
   template <typename T> T* as() const { return (T*)p; }

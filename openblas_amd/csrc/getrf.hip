@@ -1177,6 +1177,8 @@ static void lu_blocked(LuCtx<T>& c, long mn, long nb) {
   T* At = c.At;
   const long ldt = c.ldt;
   DevBuf invb[2] = {DevBuf(trsm_right_inv_bytes<T>(nb)), DevBuf(trsm_right_inv_bytes<T>(nb))};
+  invb[0].use_on(X);   // the trailing TRSMs read the inverses on the bulk stream
+  invb[1].use_on(X);
   auto factor_block = [&](long jb, long w) {
     c.sw_lo = jb;
     c.sw_hi = jb + w;

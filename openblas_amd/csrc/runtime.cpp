@@ -98,7 +98,6 @@ Runtime& rt() {
 
 hipStream_t bulk_stream() {
   Runtime& r = rt();
-  r.bulk_used = true;   // from now on pool_free orders reuse after the bulk stream's work
   static std::once_flag once;
   std::call_once(once, [&r] {
     int reserve = 32;
@@ -255,15 +254,15 @@ void* pool_alloc(size_t bytes) {
   return p;
 }
 
-void pool_free(void* p) {
+void pool_free(void* p, hipStream_t used_on) {
   if (!p) return;
   Runtime& r = *g_rt;
-  if (r.bulk_used) {
-    // the block may have been used on the bulk stream: everything queued
-    // later on the library stream (where the block is handed out again)
-    // waits for the bulk stream's work so far.  Errors are left for the next
-    // checked call (this runs in destructors).
-    if (hipEventRecord(r.ev_free, r.bulk) != hipSuccess || hipStreamWaitEvent(r.stream, r.ev_free, 0) != hipSuccess)
+  if (used_on && used_on != r.stream) {
+    // the block was used on another stream: everything queued later on the
+    // library stream (where the block is handed out again) waits for that
+    // stream's work so far.  Errors are left for the next checked call
+    // (this runs in destructors).
+    if (hipEventRecord(r.ev_free, used_on) != hipSuccess || hipStreamWaitEvent(r.stream, r.ev_free, 0) != hipSuccess)
       (void)hipGetLastError();
   }
   std::lock_guard<std::mutex> g(g_pool_mu);
