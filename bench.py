@@ -387,8 +387,24 @@ def bench_ozaki(L, n=8192, phis=(0.5, 2.0, 8.0), reps=3, refblas=None, cores=0):
             t = cpu_median(lambda: refblas.cblas_gemm(False, False, 1.0, As, B, 0.0, Cs), reps=3)
             r["cpu"] = {"dgemm_tflops": 2.0 * 1024 * n * n / t / 1e12, "cores": cores, "kind": "reference",
                         "sample": "native dgemm of the first 1024 rows of A, 1 warm-up + median of 3"}
+        # scheme II (integer moduli + CRT) on the same operands
+        oz2 = lambda: L.openblas_amd_dgemm_ozaki2(102, 111, 111, n, n, n, 1.0, dA.ptr, n, dB.ptr, n, 0.0, dC.ptr, n, 0)
+        nm = oz2()
+        t2 = [timed_ms(L, oz2) for _ in range(reps)]
+        ks2 = ktimer_profile(L, oz2, lambda nm_: ("mfma", PEAK_INT8 if nm_ == "ozaki_gemm" else PEAK_FP64))
+        dC.to_array(C)
+        ms_2 = float(np.median(t2))
+        r2 = {"moduli": nm, "native_fallback": nm < 0, "ms": ms_2, "fp64_equiv_tflops": fl / ms_2 / 1e9,
+              "speedup_vs_native": ms_64 / ms_2, "err_vs_ref_normwise": float(np.max(np.abs(C[rows] - ref) / gauge))}
+        if nm > 0:
+            r2["int8_tops"] = nm * fl / ms_2 / 1e9
+            r2["frac_of_int8_peak"] = nm * fl / (ms_2 * 1e-3) / PEAK_INT8
+            if "ozaki_gemm" in ks2:
+                r2["roofline"] = roofline_of(ks2, "ozaki_gemm")
+        r["scheme2"] = r2
         log(f"C4 ozaki phi={phi}: s={s} {ms_z:.1f} ms ({r['fp64_equiv_tflops']:.1f} TF/s eq, native {ms_64:.1f} ms) "
-            f"err={err_oz:.2e} native_err={err_64:.2e}")
+            f"err={err_oz:.2e} native_err={err_64:.2e}; scheme II: {nm} moduli {ms_2:.1f} ms "
+            f"(x{ms_64 / ms_2:.2f} vs native) err={r2['err_vs_ref_normwise']:.2e}")
         res.append(r)
         for d in (dA, dB, dC, dC64):
             d.free()

@@ -250,6 +250,20 @@ int openblas_get_num_procs(void);
 int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
                              blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
                              const double *B, blasint ldb, double beta, double *C, blasint ldc, int slices);
+/* The same product with Ozaki scheme II (Ozaki, Uchino, Imamura 2025):
+   rows of op(A) and columns of op(B) scaled to integers, one exact INT8
+   GEMM per modulus (pairwise coprime moduli <= 256), the product rebuilt
+   by the Chinese remainder theorem and rounded once to FP64.  moduli = 0
+   chooses the count from the operands' exponent spread (the typical entry
+   of each operand keeps 53 bits), else 1..30.  Returns the modulus count
+   used (0: nothing to multiply); -1 after an argument error (xerbla
+   "DGEMM ", info 15 for a bad modulus count); -2 when A or B holds a NaN or
+   an infinity (native FP64 GEMM); -n (n > 30) when the spread needs more
+   moduli than the table holds (native FP64 GEMM, FP64 accuracy kept);
+   OPENBLAS_AMD_DEVICE_ERROR after a device error. */
+int openblas_amd_dgemm_ozaki2(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
+                              blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
+                              const double *B, blasint ldb, double beta, double *C, blasint ldc, int moduli);
 /* Batched LU / LU solve / Cholesky of many independent small problems in one
    persistent launch per call (the reference has no batched LAPACK; these
    are the batched forms of interface/lapack/getrf.c:53, getrs.c:63 and

@@ -107,6 +107,8 @@ _SIGS = {
     "openblas_get_num_procs": (c_int, []),
     "openblas_amd_dgemm_ozaki": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int,
                                          c_double, _pd, c_int, c_int]),
+    "openblas_amd_dgemm_ozaki2": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_double, _pd, c_int, _pd, c_int,
+                                         c_double, _pd, c_int, c_int]),
     "openblas_amd_dgetrf_batch": (c_int, [_pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p), _pi, c_int]),
     "openblas_amd_sgetrf_batch": (c_int, [_pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p), _pi, c_int]),
     "openblas_amd_dgetrs_batch": (c_int, [c_char, _pi, _pi, POINTER(c_void_p), _pi, POINTER(c_void_p),

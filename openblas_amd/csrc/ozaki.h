@@ -9,4 +9,9 @@ namespace oba {
 int ozaki_dgemm(bool ta, bool tb, long m, long n, long k, double alpha, const double* a, long lda, const double* b,
                 long ldb, double beta, double* c, long ldc, int slices, hipStream_t s);
 
+// Scheme II (integer moduli + Chinese remainder theorem); returns the
+// modulus count used (moduli <= 0: chosen from the exponent spread).
+int ozaki2_dgemm(bool ta, bool tb, long m, long n, long k, double alpha, const double* a, long lda, const double* b,
+                 long ldb, double beta, double* c, long ldc, int moduli, hipStream_t s);
+
 }  // namespace oba

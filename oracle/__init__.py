@@ -171,3 +171,77 @@ def dmmch_ratio(transa, transb, alpha, A, B, beta, C_in, C_out, eps=None):
 
 def dbeg(reset=False):
     return lib().or_dbeg(1 if reset else 0)
+
+
+# ---- Ozaki scheme II (no reference routine; the cblas_dgemm contract) ---------
+# The published scheme (Ozaki, Uchino, Imamura, "Ozaki Scheme II: a
+# GEMM-oriented emulation of floating-point matrix multiplication using an
+# integer modular technique", 2025) restated by its definition rather than by
+# its modular arithmetic: the operands are truncated to integers, the integer
+# product is exact, and the result is rounded once.  The moduli and the
+# Chinese remainder theorem are only the device's way of forming that exact
+# product, so the oracle forms it with Python integers.
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197,
+              193, 191, 181, 179, 173, 167, 163, 157, 151, 149, 139, 137, 131, 127, 113)
+
+
+def crt_log2P(nm):
+    """log2 of the product of the first nm moduli, as the library rounds it
+    (the product rounded to FP64, then log2)."""
+    P = 1
+    for p in CRT_MODULI[:nm]:
+        P *= p
+    return float(np.log2(float(P)))
+
+
+def crt_bits(nm, K, spread_a=0.0, spread_b=0.0):
+    """(tA, tB) for nm moduli: the integer bits of each operand such that
+    K * 2^(tA+tB) <= P/2 (the balanced range of the CRT), split by the
+    spreads (equal spreads: tA = floor(total/2))."""
+    import math
+    kbits = math.ceil(math.log2(K)) + 1.0
+    total = math.floor(crt_log2P(nm) - kbits - 1e-9)
+    tA = math.floor((total + spread_a - spread_b) / 2.0)
+    tA = max(1, min(total - 1, tA))
+    return tA, total - tA
+
+
+def ozaki2(A, B, tA, tB):
+    """op(A) (M x K) times op(B) (K x N) by scheme II with tA / tB integer
+    bits: a'_ik = trunc(a_ik 2^(tA - e_i)), e_i the frexp exponent of the
+    row's largest |a| (0 for a zero row), b'_kj likewise per column; C' =
+    A'B' exactly (Python integers); C = C' 2^(e_i + f_j - tA - tB), rounded
+    once (Python's int -> float conversion rounds to nearest, ties to even).
+    For small sizes (pure Python)."""
+    import math
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    M, K = A.shape
+    N = B.shape[1]
+
+    def ex(v):
+        m = float(np.max(np.abs(v))) if v.size else 0.0
+        return math.frexp(m)[1] if m > 0 else 0
+
+    ea = [ex(A[i]) for i in range(M)]
+    fb = [ex(B[:, j]) for j in range(N)]
+    from fractions import Fraction
+
+    def trunc_scaled(x, sh):   # trunc(x 2^sh), exact (int() of a Fraction truncates toward zero)
+        return int(Fraction(x) * (Fraction(2) ** sh))
+    Ai = [[trunc_scaled(float(A[i, k]), tA - ea[i]) for k in range(K)] for i in range(M)]
+    Bj = [[trunc_scaled(float(B[k, j]), tB - fb[j]) for k in range(K)] for j in range(N)]
+    C = np.empty((M, N))
+    for i in range(M):
+        for j in range(N):
+            s = sum(x * y for x, y in zip(Ai[i], Bj[j]))
+            sh = ea[i] + fb[j] - tA - tB
+            # exact scaling of an integer by 2^sh, one rounding
+            C[i, j] = float(s * (1 << sh)) if sh >= 0 else _int_ratio_to_float(s, 1 << (-sh))
+    return C
+
+
+def _int_ratio_to_float(n, d):
+    """n / d (d a power of two) correctly rounded to FP64."""
+    from fractions import Fraction
+    return float(Fraction(n, d))

@@ -266,3 +266,44 @@ def test_ctest_grid_harness_on_reference_library():
                       P(Bc_.ctypes.data), I(Bc), D(beta), P(Cr.ctypes.data), I(n))
                     worst = max(worst, oracle.dmmch_ratio(ta, tb, alpha, A[:Ar], B[:Br], beta, C0[:m], Cr))
     assert worst < 16
+
+
+def test_ozaki2_oracle_pinned():
+    """oracle.ozaki2 (scheme II by its definition) against exact rational
+    products: with enough integer bits the truncation is exact and the
+    result is the correctly rounded exact product; with t bits the error
+    obeys the truncation bound K 2^(1-t) max|a_i| max|b_j| (per operand,
+    both operands); integer inputs are exact at modest t; a planted mistake
+    (floor instead of truncation toward zero) breaks the bound's symmetry
+    test on negative entries."""
+    from fractions import Fraction
+    import math
+    rng = np.random.default_rng(5)
+    A = rng.uniform(-1, 1, (6, 9)) * np.exp2(rng.integers(-8, 1, (6, 9)))
+    B = rng.uniform(-1, 1, (9, 5))
+    exact = [[sum(Fraction(A[i, k]) * Fraction(B[k, j]) for k in range(9)) for j in range(5)] for i in range(6)]
+    ex = np.array([[float(v) for v in row] for row in exact])
+    assert np.array_equal(oracle.ozaki2(A, B, 1200, 1200), ex)
+    for t in (20, 30, 40):
+        C = oracle.ozaki2(A, B, t, t)
+        for i in range(6):
+            for j in range(5):
+                bound = 9 * (2.0 ** (1 - t)) * 2 * np.abs(A[i]).max() * np.abs(B[:, j]).max() * 2
+                assert abs(float(Fraction(C[i, j]) - exact[i][j])) <= bound + abs(float(exact[i][j])) * 2 ** -52
+    Ai = rng.integers(-100, 101, (7, 8)).astype(np.float64)
+    Bi = rng.integers(-100, 101, (8, 3)).astype(np.float64)
+    assert np.array_equal(oracle.ozaki2(Ai, Bi, 12, 12), Ai @ Bi)
+    # the modulus bits the library splits between the operands
+    tA, tB = oracle.crt_bits(16, 500)
+    assert tA + tB == math.floor(oracle.crt_log2P(16) - 10 - 1e-9) and tA == (tA + tB) // 2
+    # planted mistake: floor() for the truncation (rounds negative entries away
+    # from zero): integer-exactness still holds, but at t = 20 the result is
+    # biased and differs from the oracle
+    def floored(A, B, t):
+        ea = [math.frexp(float(np.abs(r).max()))[1] for r in A]
+        fb = [math.frexp(float(np.abs(c).max()))[1] for c in B.T]
+        Aa = [[math.floor(Fraction(x) * Fraction(2) ** (t - ea[i])) for x in A[i]] for i in range(A.shape[0])]
+        Bb = [[math.floor(Fraction(x) * Fraction(2) ** (t - fb[j])) for x in B[:, j]] for j in range(B.shape[1])]
+        return np.array([[float(Fraction(sum(x * y for x, y in zip(Aa[i], Bb[j]))) * Fraction(2) ** (ea[i] + fb[j] - 2 * t))
+                          for j in range(B.shape[1])] for i in range(A.shape[0])])
+    assert not np.array_equal(floored(A, B, 20), oracle.ozaki2(A, B, 20, 20))
