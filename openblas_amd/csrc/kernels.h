@@ -35,9 +35,11 @@ template <typename T>
 size_t trsm_right_inv_bytes(long n);
 template <typename T>
 void trsm_right_prepare(bool lower, bool trans, bool unit, long n, const T* a, long lda, T* inv, hipStream_t s);
+// abort_flag: a device word that, once nonzero, turns the solve's kernels
+// into no-ops (a factorization that already failed).
 template <typename T>
 void trsm_right_prepared(bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b, long ldb,
-                         const T* inv, hipStream_t s);
+                         const T* inv, hipStream_t s, const int* abort_flag = nullptr);
 
 // LU with partial pivoting, in place; ipiv is a 1-based device int array.
 // Returns info (0, or first zero pivot j>0) -- lapack/getrf/getrf_single.c.

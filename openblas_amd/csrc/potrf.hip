@@ -10,7 +10,8 @@
 // MI355X design: recursion on halves (64-column grid): L21 <- A21*L11^-T is a
 // right-side TRSM, A22 -= L21*L21^T is an MFMA GEMM restricted to the lower
 // triangle (the role of the reference's SYRK kernel); 64x64 diagonal leaves
-// are factored in LDS by one workgroup.  A failing leaf raises a device flag
+// are factored by one workgroup with the rows in registers.  Large lower
+// factorizations run blocked with look-ahead (potrf_blocked).  A failing leaf raises a device flag
 // that turns every later kernel of the same call into a no-op, so the call
 // stops where the reference stops without a host round trip per leaf.
 #include "kernels.h"
@@ -20,14 +21,29 @@ namespace oba {
 
 constexpr int PO_NB = 64;
 
+__device__ inline float po_rl(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double po_rl(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
 // Factor the nb x nb diagonal block at a (lower: E(i,k) = a[i + k*lda];
 // upper: E(i,k) = a[k + i*lda], i >= k).  info/flag: global column offset j0.
+// Thread t keeps E(t%64, 16(t/64) .. 16(t/64)+15) in registers (the column
+// loop is unrolled by 16, so every register index is static).  Column j:
+// the wavefront that owns it takes the diagonal from lane j (readlane),
+// applies the failure rule, scales its column by 1/l_jj (potf2_L.c's
+// SCAL_K) and publishes it in LDS; then every thread subtracts l_ij l_kj
+// from its entries k in (j, i].  Two barriers per column; the staging tile
+// is only used to load and store coalesced in either layout.
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_leaf_kernel(int nb, T* a, long lda, bool upper, long j0, int* info) {
   __shared__ T E[PO_NB][PO_NB + 1];
+  __shared__ T col[PO_NB];
   __shared__ int s_fail;
   if (*info) return;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, ti = t & 63, g = t >> 6;
   for (int idx = t; idx < PO_NB * PO_NB; idx += 256) {
     int i = idx & 63, k = idx >> 6;
     T v = T(0);
@@ -36,42 +52,55 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int nb, T* a, long lda,
   }
   if (t == 0) s_fail = 0;
   __syncthreads();
-  int done = nb;
-  for (int j = 0; j < nb; j++) {
-    if (t == 0) {
-      T ajj = E[j][j];
-      if (!(ajj > T(0)) && !isnan(ajj)) {   // ajj <= 0: store and stop
-        s_fail = j + 1;
-      } else {
-        E[j][j] = sqrt(ajj);
+  T r[16];
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) r[kk] = E[ti][16 * g + kk];
+  int fail = 0;
+  for (int jq = 0; jq < 4; jq++) {
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = 16 * jq + jj;
+      if (j >= nb || fail) continue;   // uniform (no barrier is skipped by part of the workgroup)
+      if (g == jq) {
+        const T x = r[jj];
+        const T d = po_rl(x, j);   // the diagonal, from lane j
+        if (!(d > T(0)) && !isnan(d)) {   // ajj <= 0: stored as it is, stop
+          if (ti == 0) s_fail = j + 1;
+        } else {
+          const T ljj = sqrt(d);
+          const T rinv = T(1) / ljj;
+          const T lij = ti == j ? ljj : x * rinv;
+          if (ti >= j) r[jj] = lij;
+          col[ti] = ti > j ? lij : T(0);
+        }
       }
+      __syncthreads();
+      fail = s_fail;
+      if (fail) continue;
+      const T li = col[ti];
+#pragma unroll
+      for (int kk = 0; kk < 16; kk++) {
+        const int k = 16 * g + kk;
+        if (k > j && k <= ti) r[kk] -= li * col[k];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (s_fail) { done = j; break; }
-    const T rinv = T(1) / E[j][j];
-    for (int i = j + 1 + t; i < nb; i += 256) E[i][j] *= rinv;
-    __syncthreads();
-    // trailing lower update: E(i,k) -= E(i,j)*E(k,j), j < k <= i
-    const int rem = nb - j - 1;
-    for (int idx = t; idx < rem * rem; idx += 256) {
-      int i = j + 1 + idx % rem, k = j + 1 + idx / rem;
-      if (i >= k) E[i][k] -= E[i][j] * E[k][j];
-    }
-    __syncthreads();
   }
-  // write back columns [0, done] (the failing diagonal value included)
-  const int ncols = s_fail ? s_fail : nb;
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) E[ti][16 * g + kk] = r[kk];
+  __syncthreads();
+  // write back columns [0, ncols) (of a failing column only its diagonal)
+  const int ncols = fail ? fail : nb;
   for (int idx = t; idx < PO_NB * PO_NB; idx += 256) {
     int i = idx & 63, k = idx >> 6;
     if (i < nb && k < ncols && i >= k) {
-      if (s_fail && k == ncols - 1 && i > k) continue;   // failing column: only the diagonal
+      if (fail && k == ncols - 1 && i > k) continue;
       T v = E[i][k];
       if (upper) a[k + (long)i * lda] = v;
       else a[i + (long)k * lda] = v;
     }
   }
-  (void)done;
-  if (t == 0 && s_fail) *info = (int)(j0 + s_fail);
+  if (t == 0 && fail) *info = (int)(j0 + fail);
 }
 
 template <typename T>
@@ -99,13 +128,79 @@ static void potrf_rec(bool lower, long j0, long nc, T* a, long lda, int* info, h
   potrf_rec<T>(lower, j0 + n1, n2, a, lda, info, s);
 }
 
+// Blocked right-looking Cholesky (lower) with look-ahead, for large n, as
+// the blocked LU: the columns are cut into blocks of nb.  On the caller's
+// stream: block k's diagonal block (the recursion above), then the rows of
+// its panel that belong to block k+1 (right-side TRSM with the 64x64
+// diagonal inverses computed once) and block k+1's diagonal update; block
+// k+1 is factored next.  Meanwhile the bulk stream (CU mask, all but the
+// reserved CUs) solves the rest of the panel and updates the trailing
+// matrix (one GEMM for block k+1's column below its diagonal block, one
+// lower-restricted GEMM for the rest), and the caller's stream waits for it
+// before it touches those rows.  Every kernel takes the failure flag: after
+// a non-positive pivot the rest of the call is a no-op, as in the
+// recursion.  The trailing sums run over blocks instead of recursion
+// halves: a different association of the same products.
+template <typename T>
+static void potrf_blocked(long n, T* a, long lda, int* info, long nb, hipStream_t s) {
+  Runtime& r = rt();
+  hipStream_t M = s, X = bulk_stream();
+  DevBuf invb[2] = {DevBuf(trsm_right_inv_bytes<T>(nb)), DevBuf(trsm_right_inv_bytes<T>(nb))};
+  invb[0].use_on(X);   // read by the bulk stream's solves
+  invb[1].use_on(X);
+  auto at = [&](long i, long j) { return a + i + j * lda; };
+  potrf_rec<T>(true, 0, std::min(nb, n), a, lda, info, M);
+  bool pending = false;
+  for (long jb = 0; jb < n; jb += nb) {
+    const long w = std::min(nb, n - jb), jn = jb + w;
+    if (pending) {   // the trailing update by the previous block has finished
+      OBA_HIP(hipStreamWaitEvent(M, r.ev_b, 0));
+      pending = false;
+    }
+    if (jn >= n) break;
+    const long w1 = std::min(nb, n - jn);
+    T* inv = invb[(jb / nb) & 1].as<T>();
+    const T* l11 = at(jb, jb);
+    trsm_right_prepare<T>(true, true, false, w, l11, lda, inv, M);
+    trsm_right_prepared<T>(true, true, false, w1, w, l11, lda, at(jn, jb), lda, inv, M, info);
+    gemm<T>(false, true, w1, w1, w, T(-1), at(jn, jb), lda, at(jn, jb), lda, T(1), at(jn, jn), lda, M, 1, info);
+    const long rest = n - (jn + w1);
+    if (rest > 0) {
+      OBA_HIP(hipEventRecord(r.ev_a, M));
+      OBA_HIP(hipStreamWaitEvent(X, r.ev_a, 0));
+      trsm_right_prepared<T>(true, true, false, rest, w, l11, lda, at(jn + w1, jb), lda, inv, X, info);
+      gemm<T>(false, true, rest, w1, w, T(-1), at(jn + w1, jb), lda, at(jn, jb), lda, T(1), at(jn + w1, jn), lda, X, 0,
+              info);
+      gemm<T>(false, true, rest, rest, w, T(-1), at(jn + w1, jb), lda, at(jn + w1, jb), lda, T(1),
+              at(jn + w1, jn + w1), lda, X, 1, info);
+      OBA_HIP(hipEventRecord(r.ev_b, X));
+      pending = true;
+    }
+    potrf_rec<T>(true, jn, w1, a, lda, info, M);
+  }
+  if (pending) OBA_HIP(hipStreamWaitEvent(M, r.ev_b, 0));
+}
+
+// block width of the look-ahead driver (OPENBLAS_AMD_PO_NB; 0 turns it off);
+// it is used from 4 blocks on
+template <typename T>
+static long potrf_block_nb() {
+  // swept at n = 16384 over 128..1024: 256 for both precisions (dpotrf
+  // 87.3 / 74.2 / 77.0 / 75.4 ms, spotrf 51.9 / 46.4 / 49.0 / 50.6 ms)
+  long v = 256;
+  if (const char* e = getenv("OPENBLAS_AMD_PO_NB")) v = atol(e);
+  return v;
+}
+
 template <typename T>
 int potrf(bool lower, long n, T* a, long lda, hipStream_t s) {
   if (n <= 0) return 0;
   DevBuf ib(sizeof(int) * 64);
   int* info = ib.as<int>();
   OBA_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
-  potrf_rec<T>(lower, 0, n, a, lda, info, s);
+  const long nb = potrf_block_nb<T>();
+  if (lower && nb > 0 && n >= 4 * nb) potrf_blocked<T>(n, a, lda, info, nb, s);
+  else potrf_rec<T>(lower, 0, n, a, lda, info, s);
   int h = 0;
   OBA_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));
   OBA_HIP(hipStreamSynchronize(s));

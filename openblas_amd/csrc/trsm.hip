@@ -378,13 +378,13 @@ void trsm_right_prepare(bool lower, bool trans, bool unit, long n, const T* a, l
 
 template <typename T>
 void trsm_right_prepared(bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b, long ldb,
-                         const T* inv, hipStream_t s) {
+                         const T* inv, hipStream_t s, const int* abort_flag) {
   if (m <= 0 || n <= 0) return;
   InvCtx<T> ic;
   ic.a0 = a;
   ic.lda = lda;
   ic.inv = inv;
-  rec<T>(true, lower, trans, unit, m, n, a, lda, b, ldb, s, nullptr, ic);
+  rec<T>(true, lower, trans, unit, m, n, a, lda, b, ldb, s, abort_flag, ic);
 }
 
 template size_t trsm_right_inv_bytes<double>(long);
@@ -392,9 +392,9 @@ template size_t trsm_right_inv_bytes<float>(long);
 template void trsm_right_prepare<double>(bool, bool, bool, long, const double*, long, double*, hipStream_t);
 template void trsm_right_prepare<float>(bool, bool, bool, long, const float*, long, float*, hipStream_t);
 template void trsm_right_prepared<double>(bool, bool, bool, long, long, const double*, long, double*, long,
-                                          const double*, hipStream_t);
+                                          const double*, hipStream_t, const int*);
 template void trsm_right_prepared<float>(bool, bool, bool, long, long, const float*, long, float*, long,
-                                         const float*, hipStream_t);
+                                         const float*, hipStream_t, const int*);
 template void trsm<double>(bool, bool, bool, bool, long, long, double, const double*, long, double*, long, hipStream_t, const int*);
 template void trsm<float>(bool, bool, bool, bool, long, long, float, const float*, long, float*, long, hipStream_t, const int*);
 

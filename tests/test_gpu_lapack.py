@@ -266,6 +266,49 @@ def test_potrf_vs_oracle_sizes(oba, n):
         assert info.value == n - 2
 
 
+@pytest.mark.parametrize("tag,n", [("d", 2600), ("s", 1500)])
+def test_potrf_lookahead_blocks(oba, tag, n):
+    """Lower Cholesky from 4 blocks on (512 FP64 / 256 FP32 columns) runs the
+    blocked driver with look-ahead (trailing updates on the second stream).
+    Device-resident, the strict upper triangle NaN-poisoned (never read or
+    written), factor against numpy's within c*n*eps, reconstruction
+    L L^T = A, and a non-positive pivot in a late block: INFO = its column
+    (1-based) and the columns before it equal the factor of the leading
+    minor (potf2_L.c:71-75: the failing diagonal is stored, later columns
+    untouched by the factorization)."""
+    L = oba.lib()
+    dt = np.float64 if tag == "d" else np.float32
+    eps = EPS[dt]
+    rng = np.random.default_rng(2300 + n)
+    G = rng.standard_normal((n, 64))
+    A0 = G @ G.T + n * np.eye(n)
+    Lnp = np.linalg.cholesky(A0)
+    Ap = np.asfortranarray(np.tril(A0) + np.triu(np.full((n, n), np.nan), 1)).astype(dt)
+    f = L.dpotrf_ if tag == "d" else L.spotrf_
+    dA = oba.DeviceBuffer.from_array(Ap)
+    info = ctypes.c_int(-5)
+    f(b"L", oba.ip(n), dA.ptr, oba.ip(n), ctypes.byref(info))
+    got = dA.to_array(np.empty_like(Ap))
+    dA.free()
+    assert info.value == 0
+    assert np.isnan(np.triu(got, 1)[np.triu_indices(n, 1)]).all()
+    Lg = np.tril(got).astype(np.float64)
+    assert np.abs(Lg - Lnp).max() <= 50 * n * eps * np.abs(Lnp).max()
+    assert np.abs(Lg @ Lg.T - A0).max() <= 50 * n * eps * np.abs(A0).max()
+    # a negative pivot at column jf (0-based) in the last block but one
+    jf = n - 300
+    Ab = Ap.copy(order="F")
+    Ab[jf, jf] = -1e6
+    dA = oba.DeviceBuffer.from_array(Ab)
+    f(b"L", oba.ip(n), dA.ptr, oba.ip(n), ctypes.byref(info))
+    got = dA.to_array(np.empty_like(Ab))
+    dA.free()
+    assert info.value == jf + 1
+    Lm = np.linalg.cholesky(A0[:jf, :jf])
+    assert np.abs(np.tril(got[:jf, :jf]).astype(np.float64) - Lm).max() <= 50 * n * eps * np.abs(Lm).max()
+    assert got[jf, jf] < 0   # the failing value is stored
+
+
 def test_dsgesv_golden_vectors(oba, golden):
     L = oba.lib()
     for j in range(3):
