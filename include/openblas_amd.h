@@ -11,7 +11,8 @@
  * otherwise) and may live in host memory or in device memory allocated with
  * hipMalloc/openblas_amd_malloc: host arguments are staged to the GPU and
  * back, device arguments are used in place.  Every call is synchronous.
- * blasint is a 32-bit int (LP64).  There is no CPU fallback.
+ * blasint is a 32-bit int, or 64-bit with libopenblas_amd_ilp64.so and
+ * OPENBLAS_AMD_ILP64 defined.  There is no CPU fallback.
  *
  * Device errors (no usable gfx950 device, a failed HIP call, a bounded
  * in-kernel wait that ran out) never abort the caller: the call is skipped,
@@ -28,7 +29,16 @@
 extern "C" {
 #endif
 
+/* common.h:269-278 (INTERFACE64): blasint is 32-bit unless the ILP64 library
+   (libopenblas_amd_ilp64.so) is used, which needs OPENBLAS_AMD_ILP64 defined
+   here too; lapack_int follows (lapacke.h with LAPACK_ILP64).  The
+   extensions without a reference counterpart take 32-bit int in both. */
+#ifdef OPENBLAS_AMD_ILP64
+typedef long blasint;
+#else
 typedef int blasint;
+#endif
+typedef blasint lapack_int;
 
 /* enum values are ABI facts of cblas.h:62-66 */
 typedef enum CBLAS_ORDER     {CblasRowMajor = 101, CblasColMajor = 102} CBLAS_ORDER;
@@ -194,40 +204,52 @@ int dsposv_(const char *uplo, const blasint *n, const blasint *nrhs, double *a, 
 
 /* ---- LAPACKE (lapack-netlib/LAPACKE/include/lapacke.h) ---------------------- */
 /* lapacke.h:872 and lapacke_dgesv_work.c:35 */
-int LAPACKE_dgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb);
-int LAPACKE_dgesv_work(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb);
-int LAPACKE_sgesv(int matrix_layout, int n, int nrhs, float *a, int lda, int *ipiv, float *b, int ldb);
-int LAPACKE_sgesv_work(int matrix_layout, int n, int nrhs, float *a, int lda, int *ipiv, float *b, int ldb);
+lapack_int LAPACKE_dgesv(int matrix_layout, lapack_int n, lapack_int nrhs, double *a, lapack_int lda,
+                         lapack_int *ipiv, double *b, lapack_int ldb);
+lapack_int LAPACKE_dgesv_work(int matrix_layout, lapack_int n, lapack_int nrhs, double *a, lapack_int lda,
+                              lapack_int *ipiv, double *b, lapack_int ldb);
+lapack_int LAPACKE_sgesv(int matrix_layout, lapack_int n, lapack_int nrhs, float *a, lapack_int lda,
+                         lapack_int *ipiv, float *b, lapack_int ldb);
+lapack_int LAPACKE_sgesv_work(int matrix_layout, lapack_int n, lapack_int nrhs, float *a, lapack_int lda,
+                              lapack_int *ipiv, float *b, lapack_int ldb);
 /* lapacke.h:883 */
-int LAPACKE_dsgesv(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb,
-                   double *x, int ldx, int *iter);
+lapack_int LAPACKE_dsgesv(int matrix_layout, lapack_int n, lapack_int nrhs, double *a, lapack_int lda,
+                          lapack_int *ipiv, double *b, lapack_int ldb, double *x, lapack_int ldx,
+                          lapack_int *iter);
 /* lapacke.h:5845 (WORK and SWORK are accepted and not used) */
-int LAPACKE_dsgesv_work(int matrix_layout, int n, int nrhs, double *a, int lda, int *ipiv, double *b, int ldb,
-                        double *x, int ldx, double *work, float *swork, int *iter);
+lapack_int LAPACKE_dsgesv_work(int matrix_layout, lapack_int n, lapack_int nrhs, double *a, lapack_int lda,
+                               lapack_int *ipiv, double *b, lapack_int ldb, double *x, lapack_int ldx,
+                               double *work, float *swork, lapack_int *iter);
 /* lapacke.h:2974 and lapacke.h:8477 */
-int LAPACKE_dsposv(int matrix_layout, char uplo, int n, int nrhs, double *a, int lda, double *b, int ldb,
-                   double *x, int ldx, int *iter);
-int LAPACKE_dsposv_work(int matrix_layout, char uplo, int n, int nrhs, double *a, int lda, double *b, int ldb,
-                        double *x, int ldx, double *work, float *swork, int *iter);
+lapack_int LAPACKE_dsposv(int matrix_layout, char uplo, lapack_int n, lapack_int nrhs, double *a,
+                          lapack_int lda, double *b, lapack_int ldb, double *x, lapack_int ldx,
+                          lapack_int *iter);
+lapack_int LAPACKE_dsposv_work(int matrix_layout, char uplo, lapack_int n, lapack_int nrhs, double *a,
+                               lapack_int lda, double *b, lapack_int ldb, double *x, lapack_int ldx,
+                               double *work, float *swork, lapack_int *iter);
 /* lapacke.h:1062-1064 */
-int LAPACKE_dgetrf(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
-int LAPACKE_dgetrf_work(int matrix_layout, int m, int n, double *a, int lda, int *ipiv);
-int LAPACKE_sgetrf(int matrix_layout, int m, int n, float *a, int lda, int *ipiv);
-int LAPACKE_sgetrf_work(int matrix_layout, int m, int n, float *a, int lda, int *ipiv);
+lapack_int LAPACKE_dgetrf(int matrix_layout, lapack_int m, lapack_int n, double *a, lapack_int lda,
+                          lapack_int *ipiv);
+lapack_int LAPACKE_dgetrf_work(int matrix_layout, lapack_int m, lapack_int n, double *a, lapack_int lda,
+                               lapack_int *ipiv);
+lapack_int LAPACKE_sgetrf(int matrix_layout, lapack_int m, lapack_int n, float *a, lapack_int lda,
+                          lapack_int *ipiv);
+lapack_int LAPACKE_sgetrf_work(int matrix_layout, lapack_int m, lapack_int n, float *a, lapack_int lda,
+                               lapack_int *ipiv);
 /* lapacke.h:1098 */
-int LAPACKE_dgetrs(int matrix_layout, char trans, int n, int nrhs, const double *a, int lda, const int *ipiv,
-                   double *b, int ldb);
-int LAPACKE_dgetrs_work(int matrix_layout, char trans, int n, int nrhs, const double *a, int lda,
-                        const int *ipiv, double *b, int ldb);
-int LAPACKE_sgetrs(int matrix_layout, char trans, int n, int nrhs, const float *a, int lda, const int *ipiv,
-                   float *b, int ldb);
-int LAPACKE_sgetrs_work(int matrix_layout, char trans, int n, int nrhs, const float *a, int lda,
-                        const int *ipiv, float *b, int ldb);
+lapack_int LAPACKE_dgetrs(int matrix_layout, char trans, lapack_int n, lapack_int nrhs, const double *a,
+                          lapack_int lda, const lapack_int *ipiv, double *b, lapack_int ldb);
+lapack_int LAPACKE_dgetrs_work(int matrix_layout, char trans, lapack_int n, lapack_int nrhs, const double *a,
+                               lapack_int lda, const lapack_int *ipiv, double *b, lapack_int ldb);
+lapack_int LAPACKE_sgetrs(int matrix_layout, char trans, lapack_int n, lapack_int nrhs, const float *a,
+                          lapack_int lda, const lapack_int *ipiv, float *b, lapack_int ldb);
+lapack_int LAPACKE_sgetrs_work(int matrix_layout, char trans, lapack_int n, lapack_int nrhs, const float *a,
+                               lapack_int lda, const lapack_int *ipiv, float *b, lapack_int ldb);
 /* lapacke.h:3055-3057 */
-int LAPACKE_dpotrf(int matrix_layout, char uplo, int n, double *a, int lda);
-int LAPACKE_dpotrf_work(int matrix_layout, char uplo, int n, double *a, int lda);
-int LAPACKE_spotrf(int matrix_layout, char uplo, int n, float *a, int lda);
-int LAPACKE_spotrf_work(int matrix_layout, char uplo, int n, float *a, int lda);
+lapack_int LAPACKE_dpotrf(int matrix_layout, char uplo, lapack_int n, double *a, lapack_int lda);
+lapack_int LAPACKE_dpotrf_work(int matrix_layout, char uplo, lapack_int n, double *a, lapack_int lda);
+lapack_int LAPACKE_spotrf(int matrix_layout, char uplo, lapack_int n, float *a, lapack_int lda);
+lapack_int LAPACKE_spotrf_work(int matrix_layout, char uplo, lapack_int n, float *a, lapack_int lda);
 
 /* ---- runtime / informational (cblas.h:13-27, driver/others/xerbla.c:56) ----- */
 int xerbla_(char *name, blasint *info, blasint len);   /* weak: applications may override */
@@ -248,8 +270,8 @@ int openblas_get_num_procs(void);
    than the kernel holds (the native FP64 GEMM computed C, so the result
    keeps FP64 accuracy); OPENBLAS_AMD_DEVICE_ERROR after a device error. */
 int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
-                             blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
-                             const double *B, blasint ldb, double beta, double *C, blasint ldc, int slices);
+                             int M, int N, int K, double alpha, const double *A, int lda,
+                             const double *B, int ldb, double beta, double *C, int ldc, int slices);
 /* The same product with Ozaki scheme II (Ozaki, Uchino, Imamura 2025):
    rows of op(A) and columns of op(B) scaled to integers, one exact INT8
    GEMM per modulus (pairwise coprime moduli <= 256), the product rebuilt
@@ -262,8 +284,8 @@ int openblas_amd_dgemm_ozaki(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA
    moduli than the table holds (native FP64 GEMM, FP64 accuracy kept);
    OPENBLAS_AMD_DEVICE_ERROR after a device error. */
 int openblas_amd_dgemm_ozaki2(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE TransA, enum CBLAS_TRANSPOSE TransB,
-                              blasint M, blasint N, blasint K, double alpha, const double *A, blasint lda,
-                              const double *B, blasint ldb, double beta, double *C, blasint ldc, int moduli);
+                              int M, int N, int K, double alpha, const double *A, int lda,
+                              const double *B, int ldb, double beta, double *C, int ldc, int moduli);
 /* Batched LU / LU solve / Cholesky of many independent small problems in one
    persistent launch per call (the reference has no batched LAPACK; these
    are the batched forms of interface/lapack/getrf.c:53, getrs.c:63 and
@@ -275,20 +297,20 @@ int openblas_amd_dgemm_ozaki2(enum CBLAS_ORDER Order, enum CBLAS_TRANSPOSE Trans
    when argument k of the batched call itself is illegal (trans/uplo,
    count < 0).  Problems up to 512 rows run in the batched kernels, larger
    ones through the single-problem path. */
-int openblas_amd_dgetrf_batch(const blasint *m, const blasint *n, double *const *a, const blasint *lda,
-                              blasint *const *ipiv, blasint *info, blasint count);
-int openblas_amd_sgetrf_batch(const blasint *m, const blasint *n, float *const *a, const blasint *lda,
-                              blasint *const *ipiv, blasint *info, blasint count);
-int openblas_amd_dgetrs_batch(char trans, const blasint *n, const blasint *nrhs, double *const *a,
-                              const blasint *lda, blasint *const *ipiv, double *const *b, const blasint *ldb,
-                              blasint *info, blasint count);
-int openblas_amd_sgetrs_batch(char trans, const blasint *n, const blasint *nrhs, float *const *a,
-                              const blasint *lda, blasint *const *ipiv, float *const *b, const blasint *ldb,
-                              blasint *info, blasint count);
-int openblas_amd_dpotrf_batch(char uplo, const blasint *n, double *const *a, const blasint *lda, blasint *info,
-                              blasint count);
-int openblas_amd_spotrf_batch(char uplo, const blasint *n, float *const *a, const blasint *lda, blasint *info,
-                              blasint count);
+int openblas_amd_dgetrf_batch(const int *m, const int *n, double *const *a, const int *lda,
+                              int *const *ipiv, int *info, int count);
+int openblas_amd_sgetrf_batch(const int *m, const int *n, float *const *a, const int *lda,
+                              int *const *ipiv, int *info, int count);
+int openblas_amd_dgetrs_batch(char trans, const int *n, const int *nrhs, double *const *a,
+                              const int *lda, int *const *ipiv, double *const *b, const int *ldb,
+                              int *info, int count);
+int openblas_amd_sgetrs_batch(char trans, const int *n, const int *nrhs, float *const *a,
+                              const int *lda, int *const *ipiv, float *const *b, const int *ldb,
+                              int *info, int count);
+int openblas_amd_dpotrf_batch(char uplo, const int *n, double *const *a, const int *lda, int *info,
+                              int count);
+int openblas_amd_spotrf_batch(char uplo, const int *n, float *const *a, const int *lda, int *info,
+                              int count);
 /* device memory and synchronisation for callers that keep matrices in HBM */
 void *openblas_amd_malloc(size_t bytes);
 void openblas_amd_free(void *p);
@@ -320,9 +342,9 @@ void openblas_amd_ktimer_reset(void);
    pool.  Concurrent calls on one GPU race on that array.  A host A or B, or
    N > 32768, is reported through xerbla ("DGETRS_NOALLOC", parameter 4, 7
    or 2) with INFO = -4, -7 or -2. */
-void openblas_amd_dgetrs_noalloc(const char *trans, const blasint *n, const blasint *nrhs, const double *a,
-                                 const blasint *lda, const blasint *ipiv, double *b, const blasint *ldb,
-                                 blasint *info);
+void openblas_amd_dgetrs_noalloc(const char *trans, const int *n, const int *nrhs, const double *a,
+                                 const int *lda, const int *ipiv, double *b, const int *ldb,
+                                 int *info);
 /* last argument error reported through xerbla on this thread (0 if none);
    copies the routine name; clears the record */
 int openblas_amd_last_xerbla(char *name, int len);

@@ -15,6 +15,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libopenblas_amd.so")
+# the same entry points with 64-bit blasint / lapack_int (ILP64)
+ILP64_LIB_PATH = os.path.join(_HERE, "libopenblas_amd_ilp64.so")
 
 # ABI enum values (cblas.h:62-66)
 RowMajor, ColMajor = 101, 102

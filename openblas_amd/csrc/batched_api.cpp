@@ -120,14 +120,14 @@ void order_largest_first(std::vector<BDesc>& p) {
 }
 
 template <typename T>
-int getrf_batch_impl(const char* name, const blasint* m, const blasint* n, T* const* a, const blasint* lda,
-                     blasint* const* ipiv, blasint* info, blasint count) {
+int getrf_batch_impl(const char* name, const int* m, const int* n, T* const* a, const int* lda,
+                     int* const* ipiv, int* info, int count) {
   if (count < 0) { report_xerbla(name, 7); return -7; }
   if (count == 0) return 0;
   std::vector<BDesc> probs;
   bool reported = false;
-  for (blasint i = 0; i < count; i++) {
-    blasint e = 0;
+  for (int i = 0; i < count; i++) {
+    int e = 0;
     if (lda[i] < std::max(1, m[i])) e = 4;
     if (n[i] < 0) e = 2;
     if (m[i] < 0) e = 1;
@@ -159,9 +159,9 @@ int getrf_batch_impl(const char* name, const blasint* m, const blasint* n, T* co
 }
 
 template <typename T>
-int getrs_batch_impl(const char* name, char trans, const blasint* n, const blasint* nrhs, T* const* a,
-                     const blasint* lda, blasint* const* ipiv, T* const* b, const blasint* ldb, blasint* info,
-                     blasint count) {
+int getrs_batch_impl(const char* name, char trans, const int* n, const int* nrhs, T* const* a,
+                     const int* lda, int* const* ipiv, T* const* b, const int* ldb, int* info,
+                     int count) {
   const char tc = (char)toupper((unsigned char)trans);
   int tr = -1;
   if (tc == 'N' || tc == 'R') tr = 0;
@@ -171,8 +171,8 @@ int getrs_batch_impl(const char* name, char trans, const blasint* n, const blasi
   if (count == 0) return 0;
   std::vector<BDesc> probs;
   bool reported = false;
-  for (blasint i = 0; i < count; i++) {
-    blasint e = 0;
+  for (int i = 0; i < count; i++) {
+    int e = 0;
     if (ldb[i] < std::max(1, n[i])) e = 8;
     if (lda[i] < std::max(1, n[i])) e = 5;
     if (nrhs[i] < 0) e = 3;
@@ -207,8 +207,8 @@ int getrs_batch_impl(const char* name, char trans, const blasint* n, const blasi
 }
 
 template <typename T>
-int potrf_batch_impl(const char* name, char uplo, const blasint* n, T* const* a, const blasint* lda, blasint* info,
-                     blasint count) {
+int potrf_batch_impl(const char* name, char uplo, const int* n, T* const* a, const int* lda, int* info,
+                     int count) {
   const char uc = (char)toupper((unsigned char)uplo);
   int lower = -1;
   if (uc == 'U') lower = 0;
@@ -218,8 +218,8 @@ int potrf_batch_impl(const char* name, char uplo, const blasint* n, T* const* a,
   if (count == 0) return 0;
   std::vector<BDesc> probs;
   bool reported = false;
-  for (blasint i = 0; i < count; i++) {
-    blasint e = 0;
+  for (int i = 0; i < count; i++) {
+    int e = 0;
     if (lda[i] < std::max(1, n[i])) e = 4;
     if (n[i] < 0) e = 2;
     if (e) {
@@ -289,40 +289,40 @@ template void gemm_batch_run<float>(std::vector<BDesc>&);
 
 using namespace oba;
 
-OBA_API int openblas_amd_dgetrf_batch(const blasint* m, const blasint* n, double* const* a, const blasint* lda,
-                                      blasint* const* ipiv, blasint* info, blasint count) {
+OBA_API int openblas_amd_dgetrf_batch(const int* m, const int* n, double* const* a, const int* lda,
+                                      int* const* ipiv, int* info, int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = getrf_batch_impl<double>("DGETRF", m, n, a, lda, ipiv, info, count); });
   return r;
 }
-OBA_API int openblas_amd_sgetrf_batch(const blasint* m, const blasint* n, float* const* a, const blasint* lda,
-                                      blasint* const* ipiv, blasint* info, blasint count) {
+OBA_API int openblas_amd_sgetrf_batch(const int* m, const int* n, float* const* a, const int* lda,
+                                      int* const* ipiv, int* info, int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = getrf_batch_impl<float>("SGETRF", m, n, a, lda, ipiv, info, count); });
   return r;
 }
-OBA_API int openblas_amd_dgetrs_batch(char trans, const blasint* n, const blasint* nrhs, double* const* a,
-                                      const blasint* lda, blasint* const* ipiv, double* const* b, const blasint* ldb,
-                                      blasint* info, blasint count) {
+OBA_API int openblas_amd_dgetrs_batch(char trans, const int* n, const int* nrhs, double* const* a,
+                                      const int* lda, int* const* ipiv, double* const* b, const int* ldb,
+                                      int* info, int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = getrs_batch_impl<double>("DGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count); });
   return r;
 }
-OBA_API int openblas_amd_sgetrs_batch(char trans, const blasint* n, const blasint* nrhs, float* const* a,
-                                      const blasint* lda, blasint* const* ipiv, float* const* b, const blasint* ldb,
-                                      blasint* info, blasint count) {
+OBA_API int openblas_amd_sgetrs_batch(char trans, const int* n, const int* nrhs, float* const* a,
+                                      const int* lda, int* const* ipiv, float* const* b, const int* ldb,
+                                      int* info, int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = getrs_batch_impl<float>("SGETRS", trans, n, nrhs, a, lda, ipiv, b, ldb, info, count); });
   return r;
 }
-OBA_API int openblas_amd_dpotrf_batch(char uplo, const blasint* n, double* const* a, const blasint* lda, blasint* info,
-                                      blasint count) {
+OBA_API int openblas_amd_dpotrf_batch(char uplo, const int* n, double* const* a, const int* lda, int* info,
+                                      int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = potrf_batch_impl<double>("DPOTRF", uplo, n, a, lda, info, count); });
   return r;
 }
-OBA_API int openblas_amd_spotrf_batch(char uplo, const blasint* n, float* const* a, const blasint* lda, blasint* info,
-                                      blasint count) {
+OBA_API int openblas_amd_spotrf_batch(char uplo, const int* n, float* const* a, const int* lda, int* info,
+                                      int count) {
   int r = OBA_DEVICE_ERROR;
   guarded([&] { r = potrf_batch_impl<float>("SPOTRF", uplo, n, a, lda, info, count); });
   return r;

@@ -8,7 +8,15 @@
 #include <mutex>
 #include <stdexcept>
 
+// BLAS/LAPACK integers: 32-bit (LP64), or 64-bit in the ILP64 build
+// (OPENBLAS_AMD_ILP64, the analogue of the reference's INTERFACE64=1;
+// LAPACKE's lapack_int follows, as with LAPACK_ILP64).
+#ifdef OPENBLAS_AMD_ILP64
+typedef long blasint;
+#else
 typedef int blasint;
+#endif
+typedef blasint lapack_int;
 
 #define OBA_API extern "C" __attribute__((visibility("default")))
 
@@ -51,7 +59,7 @@ inline bool guarded(F&& f) noexcept {
 
 // BLAS/LAPACK argument-error reporting (calls the weak, user-overridable
 // xerbla_ exactly like the reference, driver/others/xerbla.c:56-64).
-void report_xerbla(const char* name, blasint info);
+void report_xerbla(const char* name, long info);
 
 // ---- runtime: one device, one main stream, one look-ahead stream ---------
 struct Runtime {

@@ -20,35 +20,78 @@ using namespace oba;
 
 namespace {
 
+// The caller's integer array (IPIV) as the kernels' 32-bit device array.
+// LP64: a device array is used in place, a host one staged.  ILP64: always
+// a device int array, converted from / to the caller's 64-bit values
+// (host or device) through the host.
 struct IntDev {
   DevBuf buf;
   int* d = nullptr;
-  int* host = nullptr;
+  blasint* host = nullptr;    // caller's host array (LP64 staging)
+  blasint* user = nullptr;    // caller's array (ILP64 conversion back)
+  bool user_dev = false;
   long n = 0;
-  void in(int* p, long count, bool copy_in) {
+  void in(blasint* p, long count, bool copy_in) {
     n = count;
     if (count <= 0) return;
-    if (is_device_ptr(p)) { d = p; return; }
-    host = p;
+    if (sizeof(blasint) == sizeof(int)) {
+      if (is_device_ptr(p)) { d = (int*)p; return; }
+      host = p;
+      buf = DevBuf(sizeof(int) * count);
+      d = buf.as<int>();
+      if (copy_in) OBA_HIP(hipMemcpyAsync(d, p, sizeof(int) * count, hipMemcpyHostToDevice, rt().stream));
+      return;
+    }
+    user = p;
+    user_dev = is_device_ptr(p);
     buf = DevBuf(sizeof(int) * count);
     d = buf.as<int>();
-    if (copy_in) OBA_HIP(hipMemcpyAsync(d, p, sizeof(int) * count, hipMemcpyHostToDevice, rt().stream));
+    if (copy_in) {
+      std::vector<blasint> w(count);
+      read_ints(p, user_dev, count, w.data());
+      std::vector<int> v(w.begin(), w.end());
+      OBA_HIP(hipMemcpyAsync(d, v.data(), sizeof(int) * count, hipMemcpyHostToDevice, rt().stream));
+      OBA_HIP(hipStreamSynchronize(rt().stream));
+    }
   }
   void out() {
-    if (!host || n <= 0) return;
-    OBA_HIP(hipMemcpyAsync(host, d, sizeof(int) * n, hipMemcpyDeviceToHost, rt().stream));
+    if (n <= 0) return;
+    if (sizeof(blasint) == sizeof(int)) {
+      if (!host) return;
+      OBA_HIP(hipMemcpyAsync(host, d, sizeof(int) * n, hipMemcpyDeviceToHost, rt().stream));
+      OBA_HIP(hipStreamSynchronize(rt().stream));
+      return;
+    }
+    if (!user) return;
+    std::vector<int> v(n);
+    OBA_HIP(hipMemcpyAsync(v.data(), d, sizeof(int) * n, hipMemcpyDeviceToHost, rt().stream));
     OBA_HIP(hipStreamSynchronize(rt().stream));
+    std::vector<blasint> w(v.begin(), v.end());
+    if (user_dev) {
+      OBA_HIP(hipMemcpyAsync(user, w.data(), sizeof(blasint) * n, hipMemcpyHostToDevice, rt().stream));
+      OBA_HIP(hipStreamSynchronize(rt().stream));
+    } else {
+      memcpy(user, w.data(), sizeof(blasint) * n);
+    }
+  }
+  // count integers of the caller's array (host or device) into out
+  static void read_ints(const blasint* p, bool dev, long count, blasint* out) {
+    if (dev) {
+      OBA_HIP(hipMemcpyAsync(out, p, sizeof(blasint) * count, hipMemcpyDeviceToHost, rt().stream));
+      OBA_HIP(hipStreamSynchronize(rt().stream));
+    } else {
+      memcpy(out, p, sizeof(blasint) * count);
+    }
   }
 };
 
 // host (or device) ipiv prefix -> device composed permutation
-void perm_from_ipiv(const int* ipiv, long n, DevBuf& buf, int** perm_dev) {
+void perm_from_ipiv(const blasint* ipiv, long n, DevBuf& buf, int** perm_dev) {
   std::vector<int> hp(n), pm;
-  if (is_device_ptr(ipiv)) {
-    OBA_HIP(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, rt().stream));
-    OBA_HIP(hipStreamSynchronize(rt().stream));
-  } else {
-    memcpy(hp.data(), ipiv, sizeof(int) * n);
+  {
+    std::vector<blasint> w(n);
+    IntDev::read_ints(ipiv, is_device_ptr(ipiv), n, w.data());
+    for (long i = 0; i < n; i++) hp[i] = (int)w[i];
   }
   ipiv_to_perm(n, hp.data(), pm);
   buf = DevBuf(sizeof(int) * n);
@@ -184,11 +227,10 @@ void laswp_impl(const blasint* N, T* a, const blasint* LDA, const blasint* K1, c
   const long inc = incx > 0 ? incx : -incx;
   const long npiv = (k1 - 1) + (k2 - k1) * inc + 1;
   std::vector<int> hp(npiv);
-  if (is_device_ptr(ipiv)) {
-    OBA_HIP(hipMemcpyAsync(hp.data(), ipiv, sizeof(int) * npiv, hipMemcpyDeviceToHost, r.stream));
-    OBA_HIP(hipStreamSynchronize(r.stream));
-  } else {
-    memcpy(hp.data(), ipiv, sizeof(int) * npiv);
+  {
+    std::vector<blasint> w(npiv);
+    IntDev::read_ints(ipiv, is_device_ptr(ipiv), npiv, w.data());
+    for (long i = 0; i < npiv; i++) hp[i] = (int)w[i];
   }
   long rows = k2;
   for (long i = k1; i <= k2; i++) rows = std::max(rows, (long)hp[(k1 - 1) + (i - k1) * inc]);
@@ -270,8 +312,8 @@ __global__ void nan_any_kernel(long m, long n, const T* a, long lda, int tri, in
 }
 
 template <typename T>
-int lapacke_gesv(const char* name, const char* wname, int layout, int n, int nrhs, T* a, int lda, int* ipiv, T* b,
-                 int ldb, bool nancheck) {
+lapack_int lapacke_gesv(const char* name, const char* wname, int layout, lapack_int n, lapack_int nrhs, T* a, lapack_int lda, lapack_int* ipiv, T* b,
+                 lapack_int ldb, bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR) {
     if (lda < n) { lapacke_xerbla(wname, -5); return -5; }
@@ -296,7 +338,7 @@ int lapacke_gesv(const char* name, const char* wname, int layout, int n, int nrh
 }
 
 template <typename T>
-int lapacke_getrf(const char* name, const char* wname, int layout, int m, int n, T* a, int lda, int* ipiv,
+lapack_int lapacke_getrf(const char* name, const char* wname, int layout, lapack_int m, lapack_int n, T* a, lapack_int lda, lapack_int* ipiv,
                   bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR && lda < n) { lapacke_xerbla(wname, -5); return -5; }
@@ -314,8 +356,8 @@ int lapacke_getrf(const char* name, const char* wname, int layout, int m, int n,
 }
 
 template <typename T>
-int lapacke_getrs(const char* name, const char* wname, int layout, char trans, int n, int nrhs, const T* a, int lda,
-                  const int* ipiv, T* b, int ldb, bool nancheck) {
+lapack_int lapacke_getrs(const char* name, const char* wname, int layout, char trans, lapack_int n, lapack_int nrhs, const T* a, lapack_int lda,
+                  const lapack_int* ipiv, T* b, lapack_int ldb, bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR) {
     if (lda < n) { lapacke_xerbla(wname, -6); return -6; }
@@ -340,7 +382,7 @@ int lapacke_getrs(const char* name, const char* wname, int layout, char trans, i
 }
 
 template <typename T>
-int lapacke_potrf(const char* name, const char* wname, int layout, char uplo, int n, T* a, int lda, bool nancheck) {
+lapack_int lapacke_potrf(const char* name, const char* wname, int layout, char uplo, lapack_int n, T* a, lapack_int lda, bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   if (layout == LAPACK_ROW_MAJOR && lda < n) { lapacke_xerbla(wname, -5); return -5; }
   (void)rt();
@@ -492,8 +534,8 @@ void dsposv_impl(const char* UPLO, const blasint* N, const blasint* NRHS, double
 // LAPACKE_dsgesv_work / LAPACKE_dsposv_work (lapacke_dsgesv_work.c:35-110,
 // lapacke_dsposv_work.c:35-115): layout check, row-major transposes, info
 // shift; the NaN pre-check belongs to the high-level wrappers only.
-int lapacke_dsxsv(bool chol, char uplo, const char* name, const char* wname, int layout, int n, int nrhs, double* a,
-                  int lda, int* ipiv, double* b, int ldb, double* x, int ldx, int* iter, bool nancheck) {
+lapack_int lapacke_dsxsv(bool chol, char uplo, const char* name, const char* wname, int layout, lapack_int n, lapack_int nrhs, double* a,
+                  lapack_int lda, lapack_int* ipiv, double* b, lapack_int ldb, double* x, lapack_int ldx, lapack_int* iter, bool nancheck) {
   if (layout != LAPACK_COL_MAJOR && layout != LAPACK_ROW_MAJOR) { lapacke_xerbla(name, -1); return -1; }
   const int sh = chol ? 1 : 0;   // dsposv has UPLO in front
   if (layout == LAPACK_ROW_MAJOR) {
@@ -547,124 +589,124 @@ OBA_API int dsposv_(const char* uplo, const blasint* N, const blasint* NRHS, dou
 }
 
 // ---- LAPACKE symbols ----------------------------------------------------------------
-OBA_API int LAPACKE_dgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgesv(int layout, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, lapack_int* ipiv, double* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_gesv<double>("LAPACKE_dgesv", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true); });
   return r;
 }
-OBA_API int LAPACKE_dgesv_work(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgesv_work(int layout, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, lapack_int* ipiv, double* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_gesv<double>("LAPACKE_dgesv_work", "LAPACKE_dgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false); });
   return r;
 }
-OBA_API int LAPACKE_sgesv(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgesv(int layout, lapack_int n, lapack_int nrhs, float* a, lapack_int lda, lapack_int* ipiv, float* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_gesv<float>("LAPACKE_sgesv", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, true); });
   return r;
 }
-OBA_API int LAPACKE_sgesv_work(int layout, int n, int nrhs, float* a, int lda, int* ipiv, float* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgesv_work(int layout, lapack_int n, lapack_int nrhs, float* a, lapack_int lda, lapack_int* ipiv, float* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_gesv<float>("LAPACKE_sgesv_work", "LAPACKE_sgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, false); });
   return r;
 }
-OBA_API int LAPACKE_dgetrf(int layout, int m, int n, double* a, int lda, int* ipiv) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgetrf(int layout, lapack_int m, lapack_int n, double* a, lapack_int lda, lapack_int* ipiv) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrf<double>("LAPACKE_dgetrf", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, true); });
   return r;
 }
-OBA_API int LAPACKE_dgetrf_work(int layout, int m, int n, double* a, int lda, int* ipiv) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgetrf_work(int layout, lapack_int m, lapack_int n, double* a, lapack_int lda, lapack_int* ipiv) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrf<double>("LAPACKE_dgetrf_work", "LAPACKE_dgetrf_work", layout, m, n, a, lda, ipiv, false); });
   return r;
 }
-OBA_API int LAPACKE_sgetrf(int layout, int m, int n, float* a, int lda, int* ipiv) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgetrf(int layout, lapack_int m, lapack_int n, float* a, lapack_int lda, lapack_int* ipiv) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrf<float>("LAPACKE_sgetrf", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, true); });
   return r;
 }
-OBA_API int LAPACKE_sgetrf_work(int layout, int m, int n, float* a, int lda, int* ipiv) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgetrf_work(int layout, lapack_int m, lapack_int n, float* a, lapack_int lda, lapack_int* ipiv) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrf<float>("LAPACKE_sgetrf_work", "LAPACKE_sgetrf_work", layout, m, n, a, lda, ipiv, false); });
   return r;
 }
-OBA_API int LAPACKE_dgetrs(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b,
-                           int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgetrs(int layout, char trans, lapack_int n, lapack_int nrhs, const double* a, lapack_int lda, const lapack_int* ipiv, double* b,
+                           lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrs<double>("LAPACKE_dgetrs", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true); });
   return r;
 }
-OBA_API int LAPACKE_dgetrs_work(int layout, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv,
-                                double* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dgetrs_work(int layout, char trans, lapack_int n, lapack_int nrhs, const double* a, lapack_int lda, const lapack_int* ipiv,
+                                double* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrs<double>("LAPACKE_dgetrs_work", "LAPACKE_dgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
                                ldb, false); });
   return r;
 }
-OBA_API int LAPACKE_sgetrs(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv, float* b,
-                           int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgetrs(int layout, char trans, lapack_int n, lapack_int nrhs, const float* a, lapack_int lda, const lapack_int* ipiv, float* b,
+                           lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrs<float>("LAPACKE_sgetrs", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b, ldb, true); });
   return r;
 }
-OBA_API int LAPACKE_sgetrs_work(int layout, char trans, int n, int nrhs, const float* a, int lda, const int* ipiv,
-                                float* b, int ldb) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_sgetrs_work(int layout, char trans, lapack_int n, lapack_int nrhs, const float* a, lapack_int lda, const lapack_int* ipiv,
+                                float* b, lapack_int ldb) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_getrs<float>("LAPACKE_sgetrs_work", "LAPACKE_sgetrs_work", layout, trans, n, nrhs, a, lda, ipiv, b,
                               ldb, false); });
   return r;
 }
-OBA_API int LAPACKE_dpotrf(int layout, char uplo, int n, double* a, int lda) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dpotrf(int layout, char uplo, lapack_int n, double* a, lapack_int lda) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_potrf<double>("LAPACKE_dpotrf", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, true); });
   return r;
 }
-OBA_API int LAPACKE_dpotrf_work(int layout, char uplo, int n, double* a, int lda) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dpotrf_work(int layout, char uplo, lapack_int n, double* a, lapack_int lda) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_potrf<double>("LAPACKE_dpotrf_work", "LAPACKE_dpotrf_work", layout, uplo, n, a, lda, false); });
   return r;
 }
-OBA_API int LAPACKE_spotrf(int layout, char uplo, int n, float* a, int lda) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_spotrf(int layout, char uplo, lapack_int n, float* a, lapack_int lda) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_potrf<float>("LAPACKE_spotrf", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, true); });
   return r;
 }
-OBA_API int LAPACKE_spotrf_work(int layout, char uplo, int n, float* a, int lda) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_spotrf_work(int layout, char uplo, lapack_int n, float* a, lapack_int lda) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] { r = lapacke_potrf<float>("LAPACKE_spotrf_work", "LAPACKE_spotrf_work", layout, uplo, n, a, lda, false); });
   return r;
 }
-OBA_API int LAPACKE_dsgesv(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb, double* x,
-                           int ldx, int* iter) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dsgesv(int layout, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, lapack_int* ipiv, double* b, lapack_int ldb, double* x,
+                           lapack_int ldx, lapack_int* iter) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] {
     r = lapacke_dsxsv(false, 'N', "LAPACKE_dsgesv", "LAPACKE_dsgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb, x,
                       ldx, iter, true);
   });
   return r;
 }
-OBA_API int LAPACKE_dsgesv_work(int layout, int n, int nrhs, double* a, int lda, int* ipiv, double* b, int ldb,
-                                double* x, int ldx, double* work, float* swork, int* iter) {
+OBA_API lapack_int LAPACKE_dsgesv_work(int layout, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, lapack_int* ipiv, double* b, lapack_int ldb,
+                                double* x, lapack_int ldx, double* work, float* swork, lapack_int* iter) {
   (void)work; (void)swork;
-  int r = OBA_DEVICE_ERROR;
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] {
     r = lapacke_dsxsv(false, 'N', "LAPACKE_dsgesv_work", "LAPACKE_dsgesv_work", layout, n, nrhs, a, lda, ipiv, b, ldb,
                       x, ldx, iter, false);
   });
   return r;
 }
-OBA_API int LAPACKE_dsposv(int layout, char uplo, int n, int nrhs, double* a, int lda, double* b, int ldb, double* x,
-                           int ldx, int* iter) {
-  int r = OBA_DEVICE_ERROR;
+OBA_API lapack_int LAPACKE_dsposv(int layout, char uplo, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, double* b, lapack_int ldb, double* x,
+                           lapack_int ldx, lapack_int* iter) {
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] {
     r = lapacke_dsxsv(true, uplo, "LAPACKE_dsposv", "LAPACKE_dsposv_work", layout, n, nrhs, a, lda, nullptr, b, ldb, x,
                       ldx, iter, true);
   });
   return r;
 }
-OBA_API int LAPACKE_dsposv_work(int layout, char uplo, int n, int nrhs, double* a, int lda, double* b, int ldb,
-                                double* x, int ldx, double* work, float* swork, int* iter) {
+OBA_API lapack_int LAPACKE_dsposv_work(int layout, char uplo, lapack_int n, lapack_int nrhs, double* a, lapack_int lda, double* b, lapack_int ldb,
+                                double* x, lapack_int ldx, double* work, float* swork, lapack_int* iter) {
   (void)work; (void)swork;
-  int r = OBA_DEVICE_ERROR;
+  lapack_int r = OBA_DEVICE_ERROR;
   guarded([&] {
     r = lapacke_dsxsv(true, uplo, "LAPACKE_dsposv_work", "LAPACKE_dsposv_work", layout, n, nrhs, a, lda, nullptr, b,
                       ldb, x, ldx, iter, false);

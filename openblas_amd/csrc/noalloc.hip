@@ -78,16 +78,16 @@ void na_zero(void* p, size_t bytes, hipStream_t s) {
 
 }  // namespace
 
-OBA_API void openblas_amd_dgetrs_noalloc(const char* TRANS, const blasint* N, const blasint* NRHS, const double* a,
-                                         const blasint* LDA, const blasint* ipiv, double* b, const blasint* LDB,
-                                         blasint* Info) {
+OBA_API void openblas_amd_dgetrs_noalloc(const char* TRANS, const int* N, const int* NRHS, const double* a,
+                                         const int* LDA, const int* ipiv, double* b, const int* LDB,
+                                         int* Info) {
   const bool ok = guarded([&] {
     const char tc = (char)toupper((unsigned char)*TRANS);
     int trans = -1;
     if (tc == 'N' || tc == 'R') trans = 0;
     if (tc == 'T' || tc == 'C') trans = 1;
     const long n = *N, nrhs = *NRHS, lda = *LDA, ldb = *LDB;
-    blasint info = 0;
+    int info = 0;
     if (ldb < std::max(1L, n)) info = 8;
     if (lda < std::max(1L, n)) info = 5;
     if (nrhs < 0) info = 3;

@@ -1049,14 +1049,14 @@ using namespace oba;
 // error, OZ_NONFINITE (-2) when an operand holds a NaN or infinity and the
 // native FP64 GEMM ran, -s (s > 12) when the operands need s slices and the
 // native FP64 GEMM ran, OBA_DEVICE_ERROR after a device error.
-OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, blasint m, blasint n, blasint k, double alpha,
-                                     const double* a, blasint lda, const double* b, blasint ldb, double beta,
-                                     double* c, blasint ldc, int slices) {
+OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, int m, int n, int k, double alpha,
+                                     const double* a, int lda, const double* b, int ldb, double beta,
+                                     double* c, int ldc, int slices) {
   int transa = -1, transb = -1;
   long M = 0, N = 0, K = k, LDA = 0, LDB = 0;
   const double *pa = nullptr, *pb = nullptr;
   auto tr = [](int t) { return (t == 111 || t == 114) ? 0 : (t == 112 || t == 113) ? 1 : -1; };
-  blasint info = 0;
+  int info = 0;
   if (order == 102) { M = m; N = n; pa = a; pb = b; LDA = lda; LDB = ldb; transa = tr(TransA); transb = tr(TransB); }
   else if (order == 101) { M = n; N = m; pa = b; pb = a; LDA = ldb; LDB = lda; transa = tr(TransB); transb = tr(TransA); }
   if (order == 101 || order == 102) {
@@ -1101,14 +1101,14 @@ OBA_API int openblas_amd_dgemm_ozaki(int order, int TransA, int TransB, blasint 
 // FP64 GEMM ran, -n (n > 30) when the spread needs more moduli than the
 // table holds and the native FP64 GEMM ran, OBA_DEVICE_ERROR after a device
 // error.
-OBA_API int openblas_amd_dgemm_ozaki2(int order, int TransA, int TransB, blasint m, blasint n, blasint k,
-                                      double alpha, const double* a, blasint lda, const double* b, blasint ldb,
-                                      double beta, double* c, blasint ldc, int moduli) {
+OBA_API int openblas_amd_dgemm_ozaki2(int order, int TransA, int TransB, int m, int n, int k,
+                                      double alpha, const double* a, int lda, const double* b, int ldb,
+                                      double beta, double* c, int ldc, int moduli) {
   int transa = -1, transb = -1;
   long M = 0, N = 0, K = k, LDA = 0, LDB = 0;
   const double *pa = nullptr, *pb = nullptr;
   auto tr = [](int t) { return (t == 111 || t == 114) ? 0 : (t == 112 || t == 113) ? 1 : -1; };
-  blasint info = 0;
+  int info = 0;
   if (order == 102) { M = m; N = n; pa = a; pb = b; LDA = lda; LDB = ldb; transa = tr(TransA); transb = tr(TransB); }
   else if (order == 101) { M = n; N = m; pa = b; pb = a; LDA = ldb; LDB = lda; transa = tr(TransB); transb = tr(TransA); }
   if (order == 101 || order == 102) {

@@ -316,7 +316,7 @@ typedef int (*xerbla_fn)(char*, blasint*, blasint);
 extern "C" int xerbla_(char* message, blasint* info, blasint len);
 namespace oba {
 
-void report_xerbla(const char* name, blasint info) {
+void report_xerbla(const char* name, long info) {
   strncpy(g_last_name, name, sizeof(g_last_name) - 1);
   g_last_info = info;
   // Route through the xerbla_ that the program itself resolves (a user
@@ -341,7 +341,7 @@ using namespace oba;
 // Weak so that applications may override it (driver/others/xerbla.c:56-64).
 extern "C" __attribute__((visibility("default"), weak)) int xerbla_(char* message, blasint* info, blasint len) {
   (void)len;
-  printf(" ** On entry to %6s parameter number %2d had an illegal value\n", message, *info);
+  printf(" ** On entry to %6s parameter number %2d had an illegal value\n", message, (int)*info);
   return 0;
 }
 
