@@ -31,7 +31,7 @@ __device__ inline float bcast_lane(float v, int k) {
 // Solves E x = b for nvec vectors, E = op(A) (left) or op(A)^T (right),
 // nb <= 64.  Vector v, element i lives at B[i*inc_e + v*inc_v].
 // eT: E(i,k) = A(k,i) (else A(i,k)); elower: E is lower triangular.
-constexpr int TRSM_VPW = 4;   // right-hand-side vectors per wavefront
+constexpr int TRSM_VPW = 2;   // right-hand-side vectors per wavefront
 
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
@@ -58,25 +58,55 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const long v0 = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * TRSM_VPW;
+  // lane i holds row i of E and its reciprocal pivot in registers; the
+  // wavefront's TRSM_VPW vectors run as interleaved chains through fully
+  // unrolled steps (static register indices; a runtime step loop reading E
+  // from LDS inside the chain took ~36 us per 64 x 1024 leaf)
+  T e[TRSM_NB];
+#pragma unroll
+  for (int k = 0; k < TRSM_NB; k++) e[k] = E[k * LDE + lane];
+  const T di = dinv[lane];
+  T x[TRSM_VPW];
+#pragma unroll
   for (int vv = 0; vv < TRSM_VPW; vv++) {
     const long v = v0 + vv;
-    if (v >= nvec) return;
-    T* b = B + v * inc_v;
-    T x = lane < nb ? b[lane * inc_e] : T(0);
-    if (elower) {
-      for (int k = 0; k < nb; k++) {
-        if (lane == k) x *= dinv[k];
-        T xk = bcast_lane(x, k);
-        if (lane > k) x -= E[k * LDE + lane] * xk;
-      }
-    } else {
-      for (int k = nb - 1; k >= 0; k--) {
-        if (lane == k) x *= dinv[k];
-        T xk = bcast_lane(x, k);
-        if (lane < k) x -= E[k * LDE + lane] * xk;
+    x[vv] = (v < nvec && lane < nb) ? B[v * inc_v + lane * inc_e] : T(0);
+  }
+  if (elower) {
+#pragma unroll
+    for (int k = 0; k < TRSM_NB; k++) {
+      if (k < nb) {
+        T xk[TRSM_VPW];
+#pragma unroll
+        for (int vv = 0; vv < TRSM_VPW; vv++) {
+          if (lane == k) x[vv] *= di;
+          xk[vv] = bcast_lane(x[vv], k);
+        }
+#pragma unroll
+        for (int vv = 0; vv < TRSM_VPW; vv++)
+          if (lane > k) x[vv] -= e[k] * xk[vv];
       }
     }
-    if (lane < nb) b[lane * inc_e] = x;
+  } else {
+#pragma unroll
+    for (int k = TRSM_NB - 1; k >= 0; k--) {
+      if (k < nb) {
+        T xk[TRSM_VPW];
+#pragma unroll
+        for (int vv = 0; vv < TRSM_VPW; vv++) {
+          if (lane == k) x[vv] *= di;
+          xk[vv] = bcast_lane(x[vv], k);
+        }
+#pragma unroll
+        for (int vv = 0; vv < TRSM_VPW; vv++)
+          if (lane < k) x[vv] -= e[k] * xk[vv];
+      }
+    }
+  }
+#pragma unroll
+  for (int vv = 0; vv < TRSM_VPW; vv++) {
+    const long v = v0 + vv;
+    if (v < nvec && lane < nb) B[v * inc_v + lane * inc_e] = x[vv];
   }
 }
 
