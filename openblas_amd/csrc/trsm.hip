@@ -196,26 +196,40 @@ __global__ __launch_bounds__(256) void trsm_inv_diag_kernel(long na, const T* __
     Ys[f * LD + g] = T(0);
   }
   __syncthreads();
-  // 16x16 diagonal blocks: thread (blk, j) computes column j of block blk
-  if (t < TRSM_NB) {
-    const int blk = t >> 4, j = t & 15, o16 = blk * 16, cj = o16 + j;
-    auto dinv = [&](int i) { return unit ? T(1) : T(1) / Ms[i * LD + i]; };
-    Ys[cj * LD + cj] = dinv(cj);
+  // 16x16 diagonal blocks: lane j < 16 of wavefront blk computes column j
+  // of block blk in registers, every step unrolled with static indices; the
+  // terms outside the column's support are skipped by selects (so an
+  // unreferenced Inf/NaN never meets a zero), the sums run in the order of
+  // the substitution: k upward from the column (lower) or from i+1 (upper).
+  if ((t & 63) < 16) {
+    const int blk = t >> 6, j = t & 15, o16 = blk * 16;
+    auto dinv = [&](int i) { return unit ? T(1) : T(1) / Ms[(o16 + i) * LD + o16 + i]; };
+    T y[16];
     if (mupper) {
-      for (int i = j - 1; i >= 0; i--) {
-        const int ri = o16 + i;
+#pragma unroll
+      for (int i = 15; i >= 0; i--) {
         T acc = T(0);
-        for (int k = i + 1; k <= j; k++) acc += Ms[ri * LD + o16 + k] * Ys[(o16 + k) * LD + cj];
-        Ys[ri * LD + cj] = -acc * dinv(ri);
+#pragma unroll
+        for (int k = i + 1; k < 16; k++) {
+          const T f = acc + Ms[(o16 + i) * LD + o16 + k] * y[k];
+          acc = k <= j ? f : acc;
+        }
+        y[i] = i > j ? T(0) : (i == j ? dinv(i) : -acc * dinv(i));
       }
     } else {
-      for (int i = j + 1; i < 16; i++) {
-        const int ri = o16 + i;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
         T acc = T(0);
-        for (int k = j; k < i; k++) acc += Ms[ri * LD + o16 + k] * Ys[(o16 + k) * LD + cj];
-        Ys[ri * LD + cj] = -acc * dinv(ri);
+#pragma unroll
+        for (int k = 0; k < i; k++) {
+          const T f = acc + Ms[(o16 + i) * LD + o16 + k] * y[k];
+          acc = k >= j ? f : acc;
+        }
+        y[i] = i < j ? T(0) : (i == j ? dinv(i) : -acc * dinv(i));
       }
     }
+#pragma unroll
+    for (int i = 0; i < 16; i++) Ys[(o16 + i) * LD + o16 + j] = y[i];
   }
   __syncthreads();
   for (int h = 16; h < TRSM_NB; h *= 2) {
