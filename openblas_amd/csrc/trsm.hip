@@ -7,12 +7,13 @@
 // (kernel/generic/trsm_ltcopy_2.c:43 INV(a) = 1/a, trsm_kernel_LT.c:95).
 //
 // Design: recursive blocking on the triangle (split in halves on a 64-row
-// grid) so that almost all flops land in large MFMA GEMM updates; the 64x64
-// diagonal leaves are solved by one wavefront per right-hand-side vector:
-// lane i owns x_i, the effective triangle sits in LDS column-major, and
-// x_k is broadcast with v_readlane after it is final (forward or backward
-// column-oriented substitution).  Right-side leaves solve rows of B, i.e.
-// the transposed system, with the same kernel.
+// grid) so that almost all flops land in large MFMA GEMM updates.  Left-side
+// leaves (up to 256 rows) are solved by wavefronts of right-hand-side
+// vectors: lane i owns x_i and row i of the current 64x64 tile of the
+// triangle, and x_k is broadcast with v_readlane after it is final (forward
+// or backward column-oriented substitution).  Right-side leaves multiply by
+// explicit inverses of the 64x64 diagonal blocks, or solve rows of B (the
+// transposed system) one row per lane.
 #include "kernels.h"
 
 namespace oba {
@@ -108,6 +109,158 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const
     const long v = v0 + vv;
     if (v < nvec && lane < nb) B[v * inc_v + lane * inc_e] = x[vv];
   }
+}
+
+// Left-side leaves of up to TRSM_LEAF_BLKS x 64 rows in one launch: the
+// 64-row blocks are solved in dependency order (forward for lower E,
+// backward for upper), each solved block's terms subtracted from the later
+// blocks of the leaf in registers (x_k broadcast by v_readlane), so the
+// recursion's two lowest GEMM levels (64 x n x 64 and 128 x n x 128 products
+// with few output tiles) and their launches disappear.  The 64x64 tiles of
+// the triangle pass through double-buffered LDS (one barrier per tile, the
+// next tile's loads in flight during the current tile's work); each lane
+// takes its row of the tile into registers.  Vectors as in trsm_leaf_kernel.
+constexpr int TRSM_LEAF_BLKS = 4;
+constexpr int TRSM_LEAF_MAX = TRSM_LEAF_BLKS * TRSM_NB;
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_leafb_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
+                                                         T* __restrict__ B, long inc_e, long inc_v, bool eT,
+                                                         bool elower, bool unit, const int* abort_flag) {
+  constexpr int LDE = TRSM_NB + 1;
+  constexpr int MB = TRSM_LEAF_BLKS;
+  constexpr int PF = TRSM_NB * TRSM_NB / 256;   // tile elements per thread
+  __shared__ T E[2][TRSM_NB * LDE];             // tile element (i, k) at E[par][k*LDE + i]
+  __shared__ T Xs[4][TRSM_VPW][TRSM_NB];        // per wavefront: a solved block's x, read as broadcasts
+  if (abort_flag && *abort_flag) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nblk = (nb + TRSM_NB - 1) / TRSM_NB;
+  const long v0 = ((long)blockIdx.x * (blockDim.x >> 6) + (t >> 6)) * TRSM_VPW;
+  // block b of the leaf in processing order q: b = q (lower) or nblk-1-q
+  auto blk_of = [&](int q) { return elower ? q : nblk - 1 - q; };
+  // the tile (i-block bi, k-block bj) of E = op(A) into registers
+  auto fetch = [&](int bi, int bj, T (&r)[PF]) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = t + u * 256, f = idx & 63, s = idx >> 6;   // f: contiguous index of A
+      const int ra = (eT ? bj : bi) * TRSM_NB + f, ca = (eT ? bi : bj) * TRSM_NB + s;
+      r[u] = (ra < nb && ca < nb) ? A[ra + (long)ca * lda] : T(0);
+    }
+  };
+  auto store = [&](int par, const T (&r)[PF]) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int idx = t + u * 256, f = idx & 63, s = idx >> 6;
+      const int i = eT ? s : f, k = eT ? f : s;
+      E[par][k * LDE + i] = r[u];
+    }
+  };
+  T x[MB][TRSM_VPW];
+#pragma unroll
+  for (int b = 0; b < MB; b++)
+#pragma unroll
+    for (int vv = 0; vv < TRSM_VPW; vv++) {
+      const long v = v0 + vv;
+      const int row = b * TRSM_NB + lane;
+      x[b][vv] = (b < nblk && v < nvec && row < nb) ? B[v * inc_v + (long)row * inc_e] : T(0);
+    }
+  auto get = [&](int b, T (&o)[TRSM_VPW]) {
+#pragma unroll
+    for (int vv = 0; vv < TRSM_VPW; vv++) {
+      T r = x[0][vv];
+#pragma unroll
+      for (int q = 1; q < MB; q++) r = b == q ? x[q][vv] : r;
+      o[vv] = r;
+    }
+  };
+  auto set = [&](int b, const T (&o)[TRSM_VPW]) {
+#pragma unroll
+    for (int q = 0; q < MB; q++)
+#pragma unroll
+      for (int vv = 0; vv < TRSM_VPW; vv++) x[q][vv] = b == q ? o[vv] : x[q][vv];
+  };
+  // tiles in order: for each block row qi, the solved blocks qj < qi, then the diagonal
+  int qi = 0, qj = 0, par = 0;
+  T pf[PF];
+  fetch(blk_of(0), blk_of(0), pf);
+  T xt[TRSM_VPW];
+  get(blk_of(0), xt);
+  while (true) {
+    store(par, pf);
+    __syncthreads();
+    const int bi = blk_of(qi), bj = blk_of(qj);
+    int ni = qi, nj = qj + 1;   // the next tile
+    if (nj > qi) { ni = qi + 1; nj = 0; }
+    const bool more = ni < nblk;
+    if (more) fetch(blk_of(ni), blk_of(nj), pf);
+    T e[TRSM_NB];
+#pragma unroll
+    for (int k = 0; k < TRSM_NB; k++) e[k] = E[par][k * LDE + lane];
+    if (qj < qi) {   // xt -= E(bi, bj) x(bj)
+      const int nk = min(TRSM_NB, nb - bj * TRSM_NB);
+      T xs[TRSM_VPW];
+      get(bj, xs);
+      // this wavefront's own LDS rows: written and read back in order by
+      // the same wavefront, no barrier
+#pragma unroll
+      for (int vv = 0; vv < TRSM_VPW; vv++) Xs[w][vv][lane] = xs[vv];
+#pragma unroll
+      for (int k = 0; k < TRSM_NB; k++) {
+        if (k < nk) {
+#pragma unroll
+          for (int vv = 0; vv < TRSM_VPW; vv++) xt[vv] -= e[k] * Xs[w][vv][k];
+        }
+      }
+    } else {         // the diagonal block, as trsm_leaf_kernel
+      const int nk = min(TRSM_NB, nb - bi * TRSM_NB);
+      const T di = (lane < nk && !unit) ? T(1) / E[par][lane * LDE + lane] : T(1);
+      if (elower) {
+#pragma unroll
+        for (int k = 0; k < TRSM_NB; k++) {
+          if (k < nk) {
+            T xk[TRSM_VPW];
+#pragma unroll
+            for (int vv = 0; vv < TRSM_VPW; vv++) {
+              if (lane == k) xt[vv] *= di;
+              xk[vv] = bcast_lane(xt[vv], k);
+            }
+#pragma unroll
+            for (int vv = 0; vv < TRSM_VPW; vv++)
+              if (lane > k) xt[vv] -= e[k] * xk[vv];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = TRSM_NB - 1; k >= 0; k--) {
+          if (k < nk) {
+            T xk[TRSM_VPW];
+#pragma unroll
+            for (int vv = 0; vv < TRSM_VPW; vv++) {
+              if (lane == k) xt[vv] *= di;
+              xk[vv] = bcast_lane(xt[vv], k);
+            }
+#pragma unroll
+            for (int vv = 0; vv < TRSM_VPW; vv++)
+              if (lane < k) xt[vv] -= e[k] * xk[vv];
+          }
+        }
+      }
+      set(bi, xt);
+      if (more) get(blk_of(ni), xt);
+    }
+    if (!more) break;
+    qi = ni;
+    qj = nj;
+    par ^= 1;
+  }
+#pragma unroll
+  for (int b = 0; b < MB; b++)
+#pragma unroll
+    for (int vv = 0; vv < TRSM_VPW; vv++) {
+      const long v = v0 + vv;
+      const int row = b * TRSM_NB + lane;
+      if (b < nblk && v < nvec && row < nb) B[v * inc_v + (long)row * inc_e] = x[b][vv];
+    }
 }
 
 // Right-side leaves: the vectors are rows of B (contiguous across vectors,
@@ -332,8 +485,12 @@ static void leaf(bool right, bool lower, bool trans, bool unit, long nb, long nr
   const int waves = 4;
   const long per_block = (long)waves * TRSM_VPW;
   long blocks = (nrhs + per_block - 1) / per_block;
-  trsm_leaf_kernel<T><<<(unsigned)blocks, 64 * waves, 0, s>>>((int)nb, nrhs, a, lda, b, inc_e, inc_v, eT, elower,
-                                                              unit, abort_flag);
+  if (nb > TRSM_NB)
+    trsm_leafb_kernel<T><<<(unsigned)blocks, 64 * waves, 0, s>>>((int)nb, nrhs, a, lda, b, inc_e, inc_v, eT, elower,
+                                                                 unit, abort_flag);
+  else
+    trsm_leaf_kernel<T><<<(unsigned)blocks, 64 * waves, 0, s>>>((int)nb, nrhs, a, lda, b, inc_e, inc_v, eT, elower,
+                                                                unit, abort_flag);
   OBA_HIP(hipGetLastError());
 }
 
@@ -342,7 +499,12 @@ template <typename T>
 static void rec(bool right, bool lower, bool trans, bool unit, long m, long n, const T* a, long lda, T* b,
                 long ldb, hipStream_t s, const int* abort_flag, const InvCtx<T>& ic) {
   const long na = right ? n : m;   // order of the triangle
-  if (na <= TRSM_NB) { leaf<T>(right, lower, trans, unit, na, right ? m : n, a, lda, b, ldb, s, abort_flag, ic); return; }
+  // left-side leaves take up to TRSM_LEAF_MAX rows (trsm_leafb_kernel);
+  // right-side leaves stay on the 64-grid of the diagonal inverses
+  if (na <= (right ? TRSM_NB : TRSM_LEAF_MAX)) {
+    leaf<T>(right, lower, trans, unit, na, right ? m : n, a, lda, b, ldb, s, abort_flag, ic);
+    return;
+  }
   long n1 = ((na / 2) + TRSM_NB - 1) / TRSM_NB * TRSM_NB;
   if (n1 >= na) n1 = na - TRSM_NB;
   const long n2 = na - n1;

@@ -253,6 +253,39 @@ def test_trsm_golden_vectors(oba, golden, tag):
         assert np.abs(X - orc).max() <= tol, key
 
 
+@pytest.mark.parametrize("tag", ["d", "s"])
+@pytest.mark.parametrize("m", [200, 256, 300])
+def test_trsm_left_multiblock_leaves(oba, tag, m):
+    """Left-side solves whose recursion leaves hold 2-4 blocks of 64 rows
+    (the multi-block leaf kernel): every uplo / trans / diag, the unstored
+    triangle NaN-poisoned, against an FP64 solve of the stored triangle."""
+    L = oba.lib()
+    dt = np.float64 if tag == "d" else np.float32
+    rng = np.random.default_rng(m)
+    nr = 37
+    f = L.cblas_dtrsm if tag == "d" else L.cblas_strsm
+    for uplo in (121, 122):
+        for tr in (111, 112):
+            for diag in (131, 132):
+                T = rng.uniform(-1, 1, (m, m)) / np.sqrt(m)   # well conditioned with a unit diagonal too
+                T[np.diag_indices(m)] = 4 + rng.uniform(0, 1, m)
+                keep = np.tril(np.ones((m, m), bool)) if uplo == 122 else np.triu(np.ones((m, m), bool))
+                Tw = np.where(keep, T, 0.0)
+                if diag == 132:
+                    Tw[np.diag_indices(m)] = 1.0
+                A = np.where(keep, T, np.nan).astype(dt, order="F")
+                if diag == 132:
+                    A[np.diag_indices(m)] = np.nan   # unit: the diagonal is not read either
+                B = np.asfortranarray(rng.uniform(-1, 1, (m, nr)).astype(dt))
+                X = B.copy(order="F")
+                f(102, 141, uplo, tr, diag, m, nr, 1.5, A.ctypes.data, m, X.ctypes.data, m)
+                op = Tw.T if tr == 112 else Tw
+                ref = np.linalg.solve(op, 1.5 * B.astype(np.float64))
+                assert not np.isnan(X).any(), (uplo, tr, diag)
+                err = np.abs(X - ref).max() / np.abs(ref).max()
+                assert err < 64 * m * EPS[dt], (uplo, tr, diag, err)
+
+
 def test_trsm_8192x1024_multiply_back(oba):
     """configs: dtrsm side=L uplo=L trans=N diag=N, M=8192, NRHS=1024, the
     triangle U[-1,1] off the diagonal and n+|U(0,1)| on it; the strictly
