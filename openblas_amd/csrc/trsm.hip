@@ -131,7 +131,10 @@ __global__ __launch_bounds__(256) void trsm_leafb_kernel(int nb, long nvec, cons
   constexpr int MB = TRSM_LEAF_BLKS;
   constexpr int PF = TRSM_NB * TRSM_NB / 256;   // tile elements per thread
   __shared__ T E[2][TRSM_NB * LDE];             // tile element (i, k) at E[par][k*LDE + i]
-  __shared__ T Xs[4][TRSM_VPW][TRSM_NB];        // per wavefront: a solved block's x, read as broadcasts
+  // per wavefront: its vectors' x, block by block (written and read back by
+  // the same wavefront only, no barrier; a register array indexed by the
+  // run-time block number was placed in scratch memory)
+  __shared__ T Xb[4][MB][TRSM_VPW][TRSM_NB];
   if (abort_flag && *abort_flag) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nblk = (nb + TRSM_NB - 1) / TRSM_NB;
@@ -155,36 +158,21 @@ __global__ __launch_bounds__(256) void trsm_leafb_kernel(int nb, long nvec, cons
       E[par][k * LDE + i] = r[u];
     }
   };
-  T x[MB][TRSM_VPW];
 #pragma unroll
   for (int b = 0; b < MB; b++)
 #pragma unroll
     for (int vv = 0; vv < TRSM_VPW; vv++) {
       const long v = v0 + vv;
       const int row = b * TRSM_NB + lane;
-      x[b][vv] = (b < nblk && v < nvec && row < nb) ? B[v * inc_v + (long)row * inc_e] : T(0);
+      Xb[w][b][vv][lane] = (b < nblk && v < nvec && row < nb) ? B[v * inc_v + (long)row * inc_e] : T(0);
     }
-  auto get = [&](int b, T (&o)[TRSM_VPW]) {
-#pragma unroll
-    for (int vv = 0; vv < TRSM_VPW; vv++) {
-      T r = x[0][vv];
-#pragma unroll
-      for (int q = 1; q < MB; q++) r = b == q ? x[q][vv] : r;
-      o[vv] = r;
-    }
-  };
-  auto set = [&](int b, const T (&o)[TRSM_VPW]) {
-#pragma unroll
-    for (int q = 0; q < MB; q++)
-#pragma unroll
-      for (int vv = 0; vv < TRSM_VPW; vv++) x[q][vv] = b == q ? o[vv] : x[q][vv];
-  };
   // tiles in order: for each block row qi, the solved blocks qj < qi, then the diagonal
   int qi = 0, qj = 0, par = 0;
   T pf[PF];
   fetch(blk_of(0), blk_of(0), pf);
   T xt[TRSM_VPW];
-  get(blk_of(0), xt);
+#pragma unroll
+  for (int vv = 0; vv < TRSM_VPW; vv++) xt[vv] = Xb[w][blk_of(0)][vv][lane];
   while (true) {
     store(par, pf);
     __syncthreads();
@@ -193,60 +181,58 @@ __global__ __launch_bounds__(256) void trsm_leafb_kernel(int nb, long nvec, cons
     if (nj > qi) { ni = qi + 1; nj = 0; }
     const bool more = ni < nblk;
     if (more) fetch(blk_of(ni), blk_of(nj), pf);
-    T e[TRSM_NB];
-#pragma unroll
-    for (int k = 0; k < TRSM_NB; k++) e[k] = E[par][k * LDE + lane];
-    if (qj < qi) {   // xt -= E(bi, bj) x(bj)
-      const int nk = min(TRSM_NB, nb - bj * TRSM_NB);
-      T xs[TRSM_VPW];
-      get(bj, xs);
-      // this wavefront's own LDS rows: written and read back in order by
-      // the same wavefront, no barrier
-#pragma unroll
-      for (int vv = 0; vv < TRSM_VPW; vv++) Xs[w][vv][lane] = xs[vv];
+    // Every step is straight-line code (a branch per step kept each step's
+    // loads from being issued early): rows of a partial block beyond its
+    // end hold exact zeros in x (the diagonal solve never updates them) and
+    // in the tile (the fetch pads with zeros), so steps past the end change
+    // nothing, whatever Inf or NaN the real values hold.
+    const T* el = &E[par][lane];
+    if (qj < qi) {   // xt -= E(bi, bj) x(bj): the tile row and x(bj) read from LDS
 #pragma unroll
       for (int k = 0; k < TRSM_NB; k++) {
-        if (k < nk) {
 #pragma unroll
-          for (int vv = 0; vv < TRSM_VPW; vv++) xt[vv] -= e[k] * Xs[w][vv][k];
-        }
+        for (int vv = 0; vv < TRSM_VPW; vv++) xt[vv] -= el[k * LDE] * Xb[w][bj][vv][k];
       }
-    } else {         // the diagonal block, as trsm_leaf_kernel
+    } else {         // the diagonal block, as trsm_leaf_kernel, the tile row in registers
       const int nk = min(TRSM_NB, nb - bi * TRSM_NB);
-      const T di = (lane < nk && !unit) ? T(1) / E[par][lane * LDE + lane] : T(1);
+      const bool live = lane < nk;
+      const T di = (live && !unit) ? T(1) / el[lane * LDE] : T(1);
+      T e[TRSM_NB];
+#pragma unroll
+      for (int k = 0; k < TRSM_NB; k++) e[k] = el[k * LDE];
       if (elower) {
 #pragma unroll
         for (int k = 0; k < TRSM_NB; k++) {
-          if (k < nk) {
-            T xk[TRSM_VPW];
+          T xk[TRSM_VPW];
 #pragma unroll
-            for (int vv = 0; vv < TRSM_VPW; vv++) {
-              if (lane == k) xt[vv] *= di;
-              xk[vv] = bcast_lane(xt[vv], k);
-            }
-#pragma unroll
-            for (int vv = 0; vv < TRSM_VPW; vv++)
-              if (lane > k) xt[vv] -= e[k] * xk[vv];
+          for (int vv = 0; vv < TRSM_VPW; vv++) {
+            if (lane == k) xt[vv] *= di;
+            xk[vv] = bcast_lane(xt[vv], k);
           }
+#pragma unroll
+          for (int vv = 0; vv < TRSM_VPW; vv++)
+            if (lane > k && live) xt[vv] -= e[k] * xk[vv];
         }
       } else {
 #pragma unroll
         for (int k = TRSM_NB - 1; k >= 0; k--) {
-          if (k < nk) {
-            T xk[TRSM_VPW];
+          T xk[TRSM_VPW];
 #pragma unroll
-            for (int vv = 0; vv < TRSM_VPW; vv++) {
-              if (lane == k) xt[vv] *= di;
-              xk[vv] = bcast_lane(xt[vv], k);
-            }
-#pragma unroll
-            for (int vv = 0; vv < TRSM_VPW; vv++)
-              if (lane < k) xt[vv] -= e[k] * xk[vv];
+          for (int vv = 0; vv < TRSM_VPW; vv++) {
+            if (lane == k) xt[vv] *= di;
+            xk[vv] = bcast_lane(xt[vv], k);
           }
+#pragma unroll
+          for (int vv = 0; vv < TRSM_VPW; vv++)
+            if (lane < k && live) xt[vv] -= e[k] * xk[vv];
         }
       }
-      set(bi, xt);
-      if (more) get(blk_of(ni), xt);
+#pragma unroll
+      for (int vv = 0; vv < TRSM_VPW; vv++) Xb[w][bi][vv][lane] = xt[vv];
+      if (more) {
+#pragma unroll
+        for (int vv = 0; vv < TRSM_VPW; vv++) xt[vv] = Xb[w][blk_of(ni)][vv][lane];
+      }
     }
     if (!more) break;
     qi = ni;
@@ -259,7 +245,7 @@ __global__ __launch_bounds__(256) void trsm_leafb_kernel(int nb, long nvec, cons
     for (int vv = 0; vv < TRSM_VPW; vv++) {
       const long v = v0 + vv;
       const int row = b * TRSM_NB + lane;
-      if (b < nblk && v < nvec && row < nb) B[v * inc_v + (long)row * inc_e] = x[b][vv];
+      if (b < nblk && v < nvec && row < nb) B[v * inc_v + (long)row * inc_e] = Xb[w][b][vv][lane];
     }
 }
 
