@@ -423,8 +423,10 @@ __global__ __launch_bounds__(256) void trsm_right_inv_kernel(int nb, long nvec, 
   T acc[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) acc[c] = T(0);
-#pragma unroll 2
-  for (int k = 0; k < nb; k++) {
+  // a static trip count (X and the inverse are zero past nb) lets each
+  // unrolled group of steps issue its LDS reads together
+#pragma unroll 8
+  for (int k = 0; k < TRSM_NB; k++) {
     const T xv = Xs[k * LDX + r];
     const T* yr = Ys + k * TRSM_NB + cg * 16;
 #pragma unroll
