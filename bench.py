@@ -16,6 +16,8 @@ Every configuration of the benchmark set is measured beside it:
       {1, 16}, with backward and forward error (b = A x_true)
   C4  INT8-slice emulated DGEMM 8192^3, phi in {0.5, 2, 8}
   C5  10,000 uneven small problems (gemm, getrf, getrs, potrf; FP64, FP32)
+  f1  dsposv_ (FP32 Cholesky + FP64 refinement) at N = 16384 on an SPD
+      system of condition kappa, beside dpotrf_ and spotrf_ alone
 Each reports TFLOP/s and a roofline record: the algorithmic flops (or HBM
 bytes) of its dominant kernel over that kernel's time, from HIP events the
 library records around its launches on the launching stream
@@ -158,6 +160,82 @@ def make_system(L, n, kappa, nrhs, seed):
     dv.free()
     dw.free()
     return dA, b
+
+
+def make_spd(L, n, kappa, seed):
+    """A = Q diag(s) Q^T (Q the product of 8 Householder reflectors, each
+    applied on both sides), s geometric from 1 to 1/kappa: symmetric
+    positive definite with 2-norm condition kappa; built on the device."""
+    rng = np.random.default_rng(seed)
+    s = kappa ** (-np.arange(n) / (n - 1))
+    A0 = np.zeros((n, n), order="F")
+    A0[np.diag_indices(n)] = s
+    dA = oa.DeviceBuffer.from_array(A0)
+    del A0
+    dv, dw = oa.DeviceBuffer(8 * n), oa.DeviceBuffer(8 * n)
+    for _ in range(8):
+        v = rng.standard_normal(n)
+        L.openblas_amd_memcpy(dv.ptr, v.ctypes.data, 8 * n, 0)
+        c = -2.0 / float(v @ v)
+        L.cblas_dgemm(102, 112, 111, n, 1, n, 1.0, dA.ptr, n, dv.ptr, n, 0.0, dw.ptr, n)
+        L.cblas_dgemm(102, 111, 112, n, n, 1, c, dv.ptr, n, dw.ptr, n, 1.0, dA.ptr, n)
+        L.cblas_dgemm(102, 111, 111, n, 1, n, 1.0, dA.ptr, n, dv.ptr, n, 0.0, dw.ptr, n)
+        L.cblas_dgemm(102, 111, 112, n, n, 1, c, dw.ptr, n, dv.ptr, n, 1.0, dA.ptr, n)
+    dv.free()
+    dw.free()
+    return dA
+
+
+def bench_cholesky(L, n, kappa, reps, refblas=None, cores=1, cfg=None):
+    """Row f1: dsposv_ (FP32 Cholesky + FP64 refinement, lower triangle) on an
+    SPD system, beside the FP64 and FP32 factorizations alone (dpotrf_,
+    spotrf_); the reference CPU dsposv_ on the same system."""
+    log(f"f1 building SPD A (n={n}, kappa={kappa:g})")
+    dA = make_spd(L, n, kappa, seed=77)
+    Ah = np.empty((n, n), order="F")
+    dA.to_array(Ah)
+    anorm = float(np.abs(Ah).sum(axis=1).max())
+    rng = np.random.default_rng(78)
+    xt = np.asfortranarray(rng.standard_normal((n, 1)))
+    dxt = oa.DeviceBuffer.from_array(xt)
+    db = oa.DeviceBuffer(8 * n)
+    L.cblas_dgemm(102, 111, 111, n, 1, n, 1.0, dA.ptr, n, dxt.ptr, n, 0.0, db.ptr, n)
+    b = db.to_array(np.empty((n, 1), order="F"))
+    dW, dX = oa.DeviceBuffer(8 * n * n), oa.DeviceBuffer(8 * n)
+    it, info = ctypes.c_int(0), ctypes.c_int(0)
+    copy_ws = lambda: L.openblas_amd_memcpy(dW.ptr, dA.ptr, 8 * n * n, 2)
+    ds = lambda: L.dsposv_(b"L", oa.ip(n), oa.ip(1), dW.ptr, oa.ip(n), db.ptr, oa.ip(n), dX.ptr, oa.ip(n), None,
+                           None, ctypes.byref(it), ctypes.byref(info))
+    ms_ds = float(np.median([timed_ms(L, ds, copy_ws) for _ in range(reps + 1)][1:]))
+    it_ds, info_ds = it.value, info.value
+    bwd, fwd = errors(L, n, dA, anorm, dX, b, xt, 1)
+    dpo = lambda: L.dpotrf_(b"L", oa.ip(n), dW.ptr, oa.ip(n), ctypes.byref(info))
+    ms_d = float(np.median([timed_ms(L, dpo, copy_ws) for _ in range(reps + 1)][1:]))
+    info_d = info.value
+    dW.free()
+    Af = np.asfortranarray(Ah.astype(np.float32))
+    dAf, dWf = oa.DeviceBuffer.from_array(Af), oa.DeviceBuffer(4 * n * n)
+    del Af
+    spo = lambda: L.spotrf_(b"L", oa.ip(n), dWf.ptr, oa.ip(n), ctypes.byref(info))
+    ms_s = float(np.median([timed_ms(L, spo, lambda: L.openblas_amd_memcpy(dWf.ptr, dAf.ptr, 4 * n * n, 2))
+                            for _ in range(reps + 1)][1:]))
+    info_s = info.value
+    for d in (dAf, dWf, dxt, db, dX, dA):
+        d.free()
+    fl = n ** 3 / 3.0
+    rec = {"n": n, "kappa": kappa, "dsposv_ms": ms_ds, "iter": it_ds, "info": info_ds, "bwd_dsposv": bwd,
+           "fwd_dsposv": fwd, "dpotrf_ms": ms_d, "dpotrf_tflops": fl / (ms_d * 1e-3) / 1e12, "dpotrf_info": info_d,
+           "spotrf_ms": ms_s, "spotrf_tflops": fl / (ms_s * 1e-3) / 1e12, "spotrf_info": info_s,
+           "dsposv_vs_dpotrf_factor_only": ms_d / ms_ds}
+    log(f"f1 dsposv {ms_ds:.1f} ms (ITER {it_ds}, bwd {bwd:.1e}, fwd {fwd:.1e}); dpotrf {ms_d:.1f} ms, "
+        f"spotrf {ms_s:.1f} ms")
+    if refblas:
+        t = cpu_median(lambda: refblas.dsposv(True, np.array(Ah, order="F"), np.array(b, order="F")), reps=3)
+        rec["cpu"] = {"dsposv_s": t, "cores": cores, "kind": "reference",
+                      "sample": f"dsposv_ N={n} NRHS=1, the full solve, 1 warm-up + median of 3, via {cfg}"}
+        log(f"f1 cpu dsposv {t:.3f} s")
+    del Ah
+    return rec
 
 
 class Solver:
@@ -611,7 +689,8 @@ def main():
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--section", default="all", choices=["all", "solve", "sweep", "dense", "ozaki", "batched"],
+    ap.add_argument("--section", default="all",
+                    choices=["all", "solve", "sweep", "dense", "ozaki", "batched", "cholesky"],
                     help="run one workload only (for profiling); prints a partial JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -638,6 +717,9 @@ def main():
         return
     if sec == "batched":
         print(json.dumps({"section": sec, "batched": bench_batched(L, reps=2, loop_sample=50)}), flush=True)
+        return
+    if sec == "cholesky":
+        print(json.dumps({"section": sec, "dsposv_16384": bench_cholesky(L, args.n, args.kappa, reps=2)}), flush=True)
         return
 
     # ---- headline: the N=16384, kappa=1e6, NRHS=1 system ----
@@ -732,6 +814,7 @@ def main():
         out_extra.update(c1c2(L, 5, refblas, cores))
         out_extra["ozaki_dgemm_8192"] = bench_ozaki(L, refblas=refblas, cores=cores)
         out_extra["batched_10000"] = bench_batched(L, refblas=refblas, cores=cores)
+        out_extra["dsposv_16384"] = bench_cholesky(L, n, args.kappa, reps=2, refblas=refblas, cores=cores, cfg=cfg)
 
     flops_solve = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * nrhs
     dom = dominant(step_k)
