@@ -10,7 +10,7 @@
 // MI355X design: recursion on halves (64-column grid): L21 <- A21*L11^-T is a
 // right-side TRSM, A22 -= L21*L21^T is an MFMA GEMM restricted to the lower
 // triangle (the role of the reference's SYRK kernel); 64x64 diagonal leaves
-// are factored by one workgroup with the rows in registers.  Large lower
+// are factored by one wavefront with the rows in registers.  Large lower
 // factorizations run blocked with look-ahead (potrf_blocked).  A failing leaf raises a device flag
 // that turns every later kernel of the same call into a no-op, so the call
 // stops where the reference stops without a host round trip per leaf.
@@ -28,86 +28,86 @@ __device__ inline double po_rl(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// Factor the nb x nb diagonal block at a (lower: E(i,k) = a[i + k*lda];
-// upper: E(i,k) = a[k + i*lda], i >= k).  info/flag: global column offset j0.
-// Thread t keeps E(t%64, 16(t/64) .. 16(t/64)+15) in registers (the column
-// loop is unrolled by 16, so every register index is static).  Column j:
-// the wavefront that owns it takes the diagonal from lane j (readlane),
-// applies the failure rule, scales its column by 1/l_jj (potf2_L.c's
-// SCAL_K) and publishes it in LDS; then every thread subtracts l_ij l_kj
-// from its entries k in (j, i].  Two barriers per column; the staging tile
-// is only used to load and store coalesced in either layout.
+// Factor the nb x nb (nb <= 64) diagonal block at a (lower: E(i,k) =
+// a[i + k*lda]; upper: E(i,k) = a[k + i*lda], i >= k); info and the failure
+// flag take the global column offset j0.  One wavefront, no barrier: lane i
+// loads row i of the block straight into registers (all loads in flight at
+// once; a staging loop through LDS waited for each load in turn, and a
+// four-wavefront form with two barriers per column took ~73 us FP32 per
+// leaf), x[k] holding column 16q + k while the column loop is in its
+// quarter q (the row is shifted down by 16 registers between quarters, so
+// every register index in the unrolled 16-column body is static).  Column
+// j: the diagonal comes from lane j (readlane); ajj <= 0 (not NaN) is
+// stored as it is and stops the leaf; otherwise the column is scaled by
+// 1/l_jj (potf2_L.c's SCAL_K), its multipliers go through LDS (written and
+// read back by the same wavefront) and every lane subtracts l_ij l_kj from
+// its entries right of j (entries above the diagonal take garbage and are
+// never stored; LDS past column 63 reads zeros).  A quarter's 16 columns
+// are final when it ends and are stored then.
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_leaf_kernel(int nb, T* a, long lda, bool upper, long j0, int* info) {
-  __shared__ T E[PO_NB][PO_NB + 1];
-  __shared__ T col[PO_NB];
-  __shared__ int s_fail;
+__global__ __launch_bounds__(64) void potrf_leaf1_kernel(int nb, T* a, long lda, bool upper, long j0, int* info) {
+  __shared__ T col[2 * PO_NB];
   if (*info) return;
-  const int t = threadIdx.x, ti = t & 63, g = t >> 6;
-  for (int idx = t; idx < PO_NB * PO_NB; idx += 256) {
-    int i = idx & 63, k = idx >> 6;
-    T v = T(0);
-    if (i < nb && k < nb && i >= k) v = upper ? a[k + (long)i * lda] : a[i + (long)k * lda];
-    E[i][k] = v;
-  }
-  if (t == 0) s_fail = 0;
-  __syncthreads();
-  T r[16];
+  const int lane = threadIdx.x;
+  const int rl = min(lane, nb - 1);   // in-bounds row for the unconditional loads
+  // row `lane` straight into registers: all 64 loads in flight at once
+  // (lower: E(i,k) = a[i + k*lda], coalesced across lanes; upper: a[k + i*lda])
+  T x[PO_NB];
 #pragma unroll
-  for (int kk = 0; kk < 16; kk++) r[kk] = E[ti][16 * g + kk];
+  for (int k = 0; k < PO_NB; k++) {
+    const int kc = min(k, nb - 1);
+    const T v = upper ? a[kc + (long)rl * lda] : a[rl + (long)kc * lda];
+    x[k] = (lane < nb && k < nb && k <= lane) ? v : T(0);
+  }
+  col[PO_NB + lane] = T(0);
   int fail = 0;
-  for (int jq = 0; jq < 4; jq++) {
+#pragma unroll 1
+  for (int q = 0; q < 4; q++) {
+    int done = 16;   // columns of this quarter that are final in x[0..15]
 #pragma unroll
     for (int jj = 0; jj < 16; jj++) {
-      const int j = 16 * jq + jj;
-      if (j >= nb || fail) continue;   // uniform (no barrier is skipped by part of the workgroup)
-      if (g == jq) {
-        const T x = r[jj];
-        const T d = po_rl(x, j);   // the diagonal, from lane j
-        if (!(d > T(0)) && !isnan(d)) {   // ajj <= 0: stored as it is, stop
-          if (ti == 0) s_fail = j + 1;
-        } else {
-          const T ljj = sqrt(d);
-          const T rinv = T(1) / ljj;
-          const T lij = ti == j ? ljj : x * rinv;
-          if (ti >= j) r[jj] = lij;
-          col[ti] = ti > j ? lij : T(0);
-        }
+      const int j = 16 * q + jj;
+      if (j >= nb || fail) { done = min(done, jj); continue; }   // uniform
+      const T d = po_rl(x[jj], j);
+      if (!(d > T(0)) && !isnan(d)) {   // ajj <= 0: stored as it is, stop
+        fail = j + 1;
+        done = jj + 1;                   // the failing column keeps its diagonal
+        continue;
       }
-      __syncthreads();
-      fail = s_fail;
-      if (fail) continue;
-      const T li = col[ti];
+      const T ljj = sqrt(d);
+      const T rinv = T(1) / ljj;
+      const T lij = lane == j ? ljj : x[jj] * rinv;
+      if (lane >= j) x[jj] = lij;
+      const T li = lane > j ? lij : T(0);
+      col[lane] = li;
 #pragma unroll
-      for (int kk = 0; kk < 16; kk++) {
-        const int k = 16 * g + kk;
-        if (k > j && k <= ti) r[kk] -= li * col[k];
+      for (int k = jj + 1; k < PO_NB; k++) x[k] -= li * col[16 * q + k];
+    }
+    // the quarter's final columns back to memory (of a failing column
+    // only its diagonal)
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int c = 16 * q + k;
+      const bool st = k < done && lane < nb && c <= lane && !(fail && c == fail - 1 && lane > c);
+      if (st) {
+        if (upper) a[c + (long)lane * lda] = x[k];
+        else a[lane + (long)c * lda] = x[k];
       }
-      __syncthreads();
     }
-  }
+    if (fail || 16 * (q + 1) >= nb) break;
 #pragma unroll
-  for (int kk = 0; kk < 16; kk++) E[ti][16 * g + kk] = r[kk];
-  __syncthreads();
-  // write back columns [0, ncols) (of a failing column only its diagonal)
-  const int ncols = fail ? fail : nb;
-  for (int idx = t; idx < PO_NB * PO_NB; idx += 256) {
-    int i = idx & 63, k = idx >> 6;
-    if (i < nb && k < ncols && i >= k) {
-      if (fail && k == ncols - 1 && i > k) continue;
-      T v = E[i][k];
-      if (upper) a[k + (long)i * lda] = v;
-      else a[i + (long)k * lda] = v;
-    }
+    for (int k = 0; k < PO_NB - 16; k++) x[k] = x[k + 16];
+#pragma unroll
+    for (int k = PO_NB - 16; k < PO_NB; k++) x[k] = T(0);
   }
-  if (t == 0 && fail) *info = (int)(j0 + fail);
+  if (lane == 0 && fail) *info = (int)(j0 + fail);
 }
 
 template <typename T>
 static void potrf_rec(bool lower, long j0, long nc, T* a, long lda, int* info, hipStream_t s) {
   T* a11 = a + j0 + j0 * lda;
   if (nc <= PO_NB) {
-    potrf_leaf_kernel<T><<<1, 256, 0, s>>>((int)nc, a11, lda, !lower, j0, info);
+    potrf_leaf1_kernel<T><<<1, 64, 0, s>>>((int)nc, a11, lda, !lower, j0, info);
     OBA_HIP(hipGetLastError());
     return;
   }

@@ -42,14 +42,22 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const
   __shared__ T E[TRSM_NB * LDE];       // E(i,k) at E[k*LDE + i]
   __shared__ T dinv[TRSM_NB];
   if (abort_flag && *abort_flag) return;
-  // consecutive threads read consecutive addresses of A in both layouts
-  for (int idx = threadIdx.x; idx < TRSM_NB * TRSM_NB; idx += blockDim.x) {
-    const int f = idx & 63, s = idx >> 6;       // f: fast (contiguous) index of A
+  // consecutive threads read consecutive addresses of A in both layouts;
+  // every load issued before the first LDS write (clamped in-bounds
+  // addresses, the padding selected after)
+  constexpr int SU = TRSM_NB * TRSM_NB / 256;
+  T sv[SU];
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = threadIdx.x + 256 * u, f = idx & 63, s = idx >> 6;   // f: fast (contiguous) index of A
+    sv[u] = A[min(f, nb - 1) + (long)min(s, nb - 1) * lda];
+  }
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = threadIdx.x + 256 * u, f = idx & 63, s = idx >> 6;
     // eT: E(i,k) = A(k,i) -> A[f + s*lda] = A(f, s) = E(s, f)
     const int i = eT ? s : f, k = eT ? f : s;
-    T v = T(0);
-    if (f < nb && s < nb) v = A[f + (long)s * lda];
-    E[k * LDE + i] = v;
+    E[k * LDE + i] = (f < nb && s < nb) ? sv[u] : T(0);
   }
   __syncthreads();
   if (threadIdx.x < TRSM_NB) {
@@ -325,11 +333,19 @@ __global__ __launch_bounds__(256) void trsm_inv_diag_kernel(long na, const T* __
   const long o = (long)blockIdx.x * TRSM_NB;
   const int nb = (int)min((long)TRSM_NB, na - o);
   const int t = threadIdx.x;
-  for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
-    const int f = idx & 63, g = idx >> 6;   // A(o+f, o+g), f contiguous
-    T v = T(0);
-    if (f < nb && g < nb) v = A[(o + f) + (o + g) * lda];
-    else if (f == g) v = T(1);   // pad with the identity
+  // staging: every load issued before the first LDS write (unconditional
+  // loads from clamped, in-bounds addresses; the padding is selected after)
+  constexpr int SU = TRSM_NB * TRSM_NB / 256;
+  T sv[SU];
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = t + 256 * u, f = idx & 63, g = idx >> 6;   // A(o+f, o+g), f contiguous
+    sv[u] = A[(o + min(f, nb - 1)) + (o + min(g, nb - 1)) * lda];
+  }
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = t + 256 * u, f = idx & 63, g = idx >> 6;
+    const T v = (f < nb && g < nb) ? sv[u] : (f == g ? T(1) : T(0));   // pad with the identity
     if (mT) Ms[g * LD + f] = v;  // M(g,f) = A(f,g)
     else Ms[f * LD + g] = v;
     Ys[f * LD + g] = T(0);
@@ -414,10 +430,22 @@ __global__ __launch_bounds__(256) void trsm_right_inv_kernel(int nb, long nvec, 
   if (abort_flag && *abort_flag) return;
   const long v0 = (long)blockIdx.x * 64;
   const int t = threadIdx.x, r = t & 63, cg = t >> 6;
-  for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
-    const int rr = idx & 63, k = idx >> 6;
-    Xs[k * LDX + rr] = (k < nb && v0 + rr < nvec) ? B[(v0 + rr) + k * ldb] : T(0);
-    Ys[idx] = inv[idx];
+  // staging: every load issued before the first LDS write (unconditional
+  // loads from clamped, in-bounds addresses; the padding is selected after)
+  constexpr int SU = TRSM_NB * TRSM_NB / 256;
+  T xv[SU], yv[SU];
+  const long vmax = nvec - 1 - v0;
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = t + 256 * u, rr = idx & 63, k = idx >> 6;
+    xv[u] = B[(v0 + min((long)rr, vmax)) + (long)min(k, nb - 1) * ldb];
+    yv[u] = inv[idx];
+  }
+#pragma unroll
+  for (int u = 0; u < SU; u++) {
+    const int idx = t + 256 * u, rr = idx & 63, k = idx >> 6;
+    Xs[k * LDX + rr] = (k < nb && v0 + rr < nvec) ? xv[u] : T(0);
+    Ys[idx] = yv[u];
   }
   __syncthreads();
   T acc[16];
