@@ -230,10 +230,22 @@ __global__ __launch_bounds__(NT) void lu_panel_kernel(T* __restrict__ At, long l
   __shared__ T s_u12[W][W];
   if (pre) {
     T* stg = &s_rows[0][0][0];   // L11 | A12 staging, free before the loop
-    // At column j0-W+k, rows j0-W .. j0+W: row k of L11 (first W), of A12
-    for (int idx = t; idx < W * 2 * W; idx += NT) {
-      const int k = idx / (2 * W), e = idx % (2 * W);
-      stg[idx] = (e < W || e - W < w) ? At[(j0 - W + e) + (j0 - W + k) * ldt] : T(0);
+    // At column j0-W+k, rows j0-W .. j0+W: row k of L11 (first W), of A12;
+    // all loads issued before the LDS writes (entries past the panel read
+    // row j0 instead, in bounds, and are zeroed)
+    static_assert((W * 2 * W) % NT == 0, "the staging divides over the workgroup");
+    constexpr int SU = W * 2 * W / NT;
+    T sv[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int idx = t + u * NT, k = idx / (2 * W), e = idx % (2 * W);
+      const int er = (e < W || e - W < w) ? e : W;
+      sv[u] = At[(j0 - W + er) + (j0 - W + k) * ldt];
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int idx = t + u * NT, e = idx % (2 * W);
+      stg[idx] = (e < W || e - W < w) ? sv[u] : T(0);
     }
     __syncthreads();
     if (t < W) {   // column t of U12: forward substitution with unit L11
@@ -655,9 +667,20 @@ __global__ __launch_bounds__(L2_NT) void lu_panel2_kernel(T* __restrict__ At, lo
   // pre: the node's U12 = L11^{-1} A12 and each row's x -= l(row) U12 (see
   // lu_panel_kernel); the k loop stays rolled to keep the code small
   if (pre) {
-    for (int idx = t; idx < W * 2 * W; idx += NT) {
-      const int k = idx / (2 * W), e = idx % (2 * W);
-      s_stage[idx] = (e < W || e - W < w) ? At[(j0 - W + e) + (j0 - W + k) * ldt] : T(0);
+    // all loads before the LDS writes, as in lu_panel_kernel
+    static_assert((W * 2 * W) % NT == 0, "the staging divides over the workgroup");
+    constexpr int SU = W * 2 * W / NT;
+    T sv[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int idx = t + u * NT, k = idx / (2 * W), e = idx % (2 * W);
+      const int er = (e < W || e - W < w) ? e : W;
+      sv[u] = At[(j0 - W + er) + (j0 - W + k) * ldt];
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int idx = t + u * NT, e = idx % (2 * W);
+      s_stage[idx] = (e < W || e - W < w) ? sv[u] : T(0);
     }
     __syncthreads();
     if (t < W) {
