@@ -973,32 +973,42 @@ template <typename T, int W>
 __global__ __launch_bounds__(256) void lu_swap_kernel(T* __restrict__ At, long ldt, long lo, long hi, long ex_lo,
                                                       long ex_hi, long j0, long mp, const int* __restrict__ moves) {
   constexpr int RB = 64;                       // At rows per workgroup
+  constexpr int MU = 2 * W * RB / 256;         // gathers per thread at most
+  static_assert(2 * W <= 256 && (2 * W * RB) % 256 == 0, "one move per thread; whole gather rounds");
   __shared__ int sd[2 * W], ss[2 * W];
-  __shared__ int s_cnt;
-  __shared__ T vals[2 * W][RB];
-  if (threadIdx.x == 0) s_cnt = min(max(moves[0], 0), 2 * W);
-  __syncthreads();
-  const int cnt = s_cnt;
-  if ((int)threadIdx.x < cnt) {
-    int d = moves[1 + 2 * threadIdx.x], sr = moves[2 + 2 * threadIdx.x];
-    if (d < 0 || d >= mp || sr < 0 || sr >= mp) d = sr = 0;   // never index outside the panel rows
-    sd[threadIdx.x] = d;
-    ss[threadIdx.x] = sr;
+  // the count and this thread's move in one round trip
+  const int t = threadIdx.x;
+  const int c0 = moves[0];
+  int d = 0, sr = 0;
+  if (t < 2 * W) {
+    d = moves[1 + 2 * t];
+    sr = moves[2 + 2 * t];
+  }
+  const int cnt = min(max(c0, 0), 2 * W);
+  if (t < 2 * W) {
+    if (t >= cnt || d < 0 || d >= mp || sr < 0 || sr >= mp) d = sr = 0;   // never index outside the panel rows
+    sd[t] = d;
+    ss[t] = sr;
   }
   __syncthreads();
   const long r0 = lo + (long)blockIdx.x * RB;
-  // gather every source first (coalesced along the rows of At) ...
-  for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
-    const int k = idx / RB, rr = idx % RB;
+  // gather every source first (coalesced along the rows of At; all of a
+  // thread's loads in flight together) ...
+  T v[MU];
+#pragma unroll
+  for (int u = 0; u < MU; u++) {
+    const int idx = t + 256 * u, k = idx / RB, rr = idx % RB;
     const long r = r0 + rr;
-    if (r < hi && !(r >= ex_lo && r < ex_hi)) vals[k][rr] = At[r + (j0 + ss[k]) * ldt];
+    const bool ok = idx < cnt * RB && r < hi && !(r >= ex_lo && r < ex_hi);
+    v[u] = ok ? At[r + (j0 + ss[k]) * ldt] : T(0);
   }
-  __syncthreads();
+  __syncthreads();   // every read before any write: a move's destination can be another's source
   // ... then scatter: the moves are a permutation of the touched columns
-  for (int idx = threadIdx.x; idx < cnt * RB; idx += blockDim.x) {
-    const int k = idx / RB, rr = idx % RB;
+#pragma unroll
+  for (int u = 0; u < MU; u++) {
+    const int idx = t + 256 * u, k = idx / RB, rr = idx % RB;
     const long r = r0 + rr;
-    if (r < hi && !(r >= ex_lo && r < ex_hi)) At[r + (j0 + sd[k]) * ldt] = vals[k][rr];
+    if (idx < cnt * RB && r < hi && !(r >= ex_lo && r < ex_hi)) At[r + (j0 + sd[k]) * ldt] = v[u];
   }
 }
 
