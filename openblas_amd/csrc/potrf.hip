@@ -185,9 +185,11 @@ static void potrf_blocked(long n, T* a, long lda, int* info, long nb, hipStream_
 // it is used from 4 blocks on
 template <typename T>
 static long potrf_block_nb() {
-  // swept at n = 16384 over 128..1024: 256 for both precisions (dpotrf
-  // 87.3 / 74.2 / 77.0 / 75.4 ms, spotrf 51.9 / 46.4 / 49.0 / 50.6 ms)
-  long v = 256;
+  // swept at n = 16384 with the one-wavefront leaves (the chain no longer
+  // the bound, the trailing updates prefer a deeper K): 256 / 320 / 384 /
+  // 448 / 512 / 640 / 768 gave dpotrf 70.2 / 65.0 / 61.8 / 62.0 / 71.8 /
+  // 60.5 / 64.8 ms, spotrf 41.3 / 39.3 / 38.6 / 38.3 / 43.0 / 39.5 / 41.5
+  long v = 384;
   if (const char* e = getenv("OPENBLAS_AMD_PO_NB")) v = atol(e);
   return v;
 }
