@@ -387,29 +387,43 @@ __global__ __launch_bounds__(256) void trsm_inv_diag_kernel(long na, const T* __
     for (int i = 0; i < 16; i++) Ys[(o16 + i) * LD + o16 + j] = y[i];
   }
   __syncthreads();
-  for (int h = 16; h < TRSM_NB; h *= 2) {
-    const int npair = TRSM_NB / (2 * h);
+  // joins 16 -> 32 -> 64 with the block size a compile-time constant, so
+  // every loop is unrolled and a thread's LDS reads issue together (same
+  // operations, same order)
+  auto join = [&](auto hc) __attribute__((always_inline)) {
+    constexpr int h = decltype(hc)::value;
+    constexpr int npair = TRSM_NB / (2 * h);
+    constexpr int PER = npair * h * h / 256;   // outputs per thread
+    static_assert(PER * 256 == npair * h * h, "whole rounds");
     // T = M_off * Y_near   (upper: M12 Y22; lower: M21 Y11)
-    for (int idx = t; idx < npair * h * h; idx += 256) {
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int idx = t + 256 * u;
       const int p = idx / (h * h), r = (idx / h) % h, c = idx % h, po = p * 2 * h;
       const int mr = mupper ? po + r : po + h + r, mc0 = mupper ? po + h : po;   // M rows / first column
       const int yc = mupper ? po + h + c : po + c;                                  // Y column
       T acc = T(0);
+#pragma unroll
       for (int k = 0; k < h; k++) acc += Ms[mr * LD + mc0 + k] * Ys[(mc0 + k) * LD + yc];
       Ts[(po + r) * LD + c] = acc;
     }
     __syncthreads();
     // Y_off = -Y_far * T   (upper: Y12 = -Y11 T; lower: Y21 = -Y22 T)
-    for (int idx = t; idx < npair * h * h; idx += 256) {
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int idx = t + 256 * u;
       const int p = idx / (h * h), r = (idx / h) % h, c = idx % h, po = p * 2 * h;
       const int yr = mupper ? po + r : po + h + r, yk0 = mupper ? po : po + h;
       T acc = T(0);
+#pragma unroll
       for (int k = 0; k < h; k++) acc += Ys[yr * LD + yk0 + k] * Ts[(po + k) * LD + c];
       const int oc = mupper ? po + h + c : po + c;
       Ys[yr * LD + oc] = -acc;
     }
     __syncthreads();
-  }
+  };
+  join(std::integral_constant<int, 16>{});
+  join(std::integral_constant<int, 32>{});
   T* out = inv + (long)blockIdx.x * TRSM_NB * TRSM_NB;
   for (int idx = t; idx < TRSM_NB * TRSM_NB; idx += 256) {
     const int i = idx >> 6, j = idx & 63;
