@@ -32,7 +32,9 @@ __device__ inline float bcast_lane(float v, int k) {
 // Solves E x = b for nvec vectors, E = op(A) (left) or op(A)^T (right),
 // nb <= 64.  Vector v, element i lives at B[i*inc_e + v*inc_v].
 // eT: E(i,k) = A(k,i) (else A(i,k)); elower: E is lower triangular.
-constexpr int TRSM_VPW = 2;   // right-hand-side vectors per wavefront
+// right-hand-side vectors per wavefront (more workgroups beat interleaved
+// chains: DTRSM 8192 x 1024 3.52 ms with 2, 3.21 ms with 1)
+constexpr int TRSM_VPW = 1;
 
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const T* __restrict__ A, long lda,
@@ -68,8 +70,8 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int nb, long nvec, const
   const int lane = threadIdx.x & 63;
   const long v0 = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * TRSM_VPW;
   // lane i holds row i of E and its reciprocal pivot in registers; the
-  // wavefront's TRSM_VPW vectors run as interleaved chains through fully
-  // unrolled steps (static register indices; a runtime step loop reading E
+  // wavefront's TRSM_VPW vectors (interleaved chains when more than one)
+  // run through fully unrolled steps (static register indices; a runtime step loop reading E
   // from LDS inside the chain took ~36 us per 64 x 1024 leaf)
   T e[TRSM_NB];
 #pragma unroll
