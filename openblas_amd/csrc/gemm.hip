@@ -486,8 +486,12 @@ void gemm(bool ta, bool tb, long m, long n, long k, T alpha, const T* a, long ld
   constexpr int BK_BIG = sizeof(T) == 8 ? 4 : 8;
   constexpr int BK_SMALL = sizeof(T) == 8 ? 8 : 16;
   const long tiles64 = ((m + 63) / 64) * ((n + 63) / 64);
-  if (small && tri == 0 && abort_flag == nullptr && k >= 1024 && 8 * tiles64 >= 7 * rt().num_cu &&
-      tiles64 <= rt().num_cu) {
+  const bool sliced_deep = k >= 1024 && 8 * tiles64 >= 7 * rt().num_cu && tiles64 <= rt().num_cu;
+  // few tiles and a moderate K (no split-K path below 1024): the four wave
+  // groups per tile hide each other's latencies (the recursive TRSM's
+  // updates, 256..512 x n x 256..512)
+  const bool sliced_mid = k >= 256 && k < 1024 && 2 * tiles64 <= rt().num_cu;
+  if (small && tri == 0 && abort_flag == nullptr && (sliced_deep || sliced_mid)) {
     // about one 64x64 tile per CU, deep K: one workgroup of 4 wave groups per
     // tile, each group a quarter of K, partials summed in LDS (4 wavefronts
     // per SIMD instead of 1; no workspace, so any stream).  MI355X, 1024^3:
