@@ -208,12 +208,19 @@ def test_ilp64_matches_lp64_bitwise(oba, L64):
     L64.dgemm_(b"N", b"N", lp(m), lp(nn), lp(k), byref(c_double(1.5)), Ag.ctypes.data, lp(m), Bg.ctypes.data, lp(k),
                byref(c_double(-0.5)), C64f.ctypes.data, lp(m))
     assert np.array_equal(C32, C64f)
-    Cb = Cg.copy(order="F")
+    # the group-batched form (its own engine: compared with the LP64 batch call)
+    Cb, Cb32 = Cg.copy(order="F"), Cg.copy(order="F")
     i64a = lambda *v: (c_long * len(v))(*v)  # noqa: E731
+    i32a = lambda *v: (c_int * len(v))(*v)  # noqa: E731
     L64.cblas_dgemm_batch(102, (c_int * 1)(111), (c_int * 1)(111), i64a(m), i64a(nn), i64a(k), (c_double * 1)(1.5),
                           (c_void_p * 1)(Ag.ctypes.data), i64a(m), (c_void_p * 1)(Bg.ctypes.data), i64a(k),
                           (c_double * 1)(-0.5), (c_void_p * 1)(Cb.ctypes.data), i64a(m), 1, i64a(1))
-    assert np.array_equal(C32, Cb)
+    L.cblas_dgemm_batch(102, i32a(111), i32a(111), i32a(m), i32a(nn), i32a(k), (c_double * 1)(1.5),
+                        (c_void_p * 1)(Ag.ctypes.data), i32a(m), (c_void_p * 1)(Bg.ctypes.data), i32a(k),
+                        (c_double * 1)(-0.5), (c_void_p * 1)(Cb32.ctypes.data), i32a(m), 1, i32a(1))
+    assert np.array_equal(Cb32, Cb)
+    assert np.abs(Cb - C32).max() <= 64 * k * np.finfo(np.float64).eps * (1.5 * np.abs(Ag) @ np.abs(Bg)
+                                                                           + 0.5 * np.abs(Cg)).max()
     # TRSM
     T = np.asfortranarray(np.tril(rng.standard_normal((m, m))) + 4 * np.eye(m))
     X32, X64 = Cg.copy(order="F"), Cg.copy(order="F")
